@@ -1,0 +1,178 @@
+/*
+ * ostr_hip.h - C ABI of libostr_hip.so, the gfx950 HIP implementation of the
+ * OSTR (ordered-subsets transmission) DBT reconstruction path.
+ *
+ * Every entry point takes plain pointers and sizes. Image and sinogram pointers
+ * are DEVICE pointers owned by the caller (for example torch.Tensor.data_ptr()
+ * on a ROCm build); the library never frees them. `stream` is a hipStream_t
+ * passed as void* (0 = the null stream). All calls are asynchronous on that
+ * stream unless stated otherwise. Return value: 0 on success, a negative
+ * OSTR_E* code on failure; ostr_last_error() gives the message. A handle is not
+ * thread-safe; separate handles are independent (no process-global geometry).
+ *
+ * Layouts (same as the reference):
+ *   volume   [nz][ny][nx], x fastest            (Tomo_OSTR.cu:1604)
+ *   sinogram [view slot][nbx][nby], nby fastest (Tomo_OSTR.cu:1365)
+ * A "slot" is a position in the view ordering table `index` (slot a uses the
+ * angle angles[index[a]]), exactly like d_index in the reference launchers.
+ */
+#ifndef OSTR_HIP_H
+#define OSTR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Geometry; mirrors the reference's __constant__ globals (Tomo_OSTR.cu:81-115,
+ * datafile keys Tomo_OSTR.cu:2578-2650). Lengths in mm. */
+typedef struct ostr_geom {
+    int32_t nx, ny, nz;        /* IMGSIZx/y/z                                   */
+    float   dx, dy, dz;        /* Vsize_x/y/z                                   */
+    float   x_p0, y_p0, z_p0;  /* corner of voxel (0,0,0)                       */
+    int32_t nbx, nby;          /* BINSx, BINSy                                  */
+    float   dbx, dby;          /* Bsize_x, Bsize_y                              */
+    float   x_d0, y_d0;        /* corner of detector bin (0,0)                  */
+    float   detector_r;        /* detectorR: flat detector plane z = -detectorR */
+    float   source_r;          /* sourceR: source at (R sin t, sourceY, R cos t) */
+    float   source_y;          /* sourceY                                       */
+} ostr_geom;
+
+typedef struct ostr_handle ostr_handle;
+
+/* error codes */
+#define OSTR_OK            0
+#define OSTR_E_ARG        -1  /* invalid argument / bad configuration          */
+#define OSTR_E_HIP        -2  /* HIP runtime error                             */
+#define OSTR_E_RANGE      -3  /* view range or size out of bounds              */
+#define OSTR_E_UNSUPPORTED -4 /* dtype / mode not supported                    */
+
+/* storage dtypes for images and sinograms */
+#define OSTR_F32  0
+#define OSTR_BF16 1
+#define OSTR_F16  2
+
+/* gamma (denominator row term) modes */
+#define OSTR_GAMMA_RAY_LENGTH 0 /* gamma = |S - D| for rays that meet the box (Tomo_OSTR.cu:1172) */
+#define OSTR_GAMMA_ROW_SUM    1 /* gamma = sum_j a_ij (ostr.ipynb normprj, raw line 815)         */
+
+/* prior potentials */
+#define OSTR_PRIOR_Q2        0 /* quadratic, G_Q2_prior (Tomo_OSTR.cu:118-184)                    */
+#define OSTR_PRIOR_LANGE     1 /* Lange/"fair", G_Fessler_prior (Tomo_OSTR.cu:187-255)            */
+#define OSTR_PRIOR_HYPERBOLA 2 /* hyperbola potential of G_Huber_prior (Tomo_OSTR.cu:258-325)
+                                  with the curvature and z weighting of G_Fessler_prior          */
+
+/* ---------------------------------------------------------------- handle -- */
+
+/* Create a handle on the current HIP device. `angles_rad` holds n_views
+ * float32 angles in radians (the reference's angle file, Tomo_OSTR_gold.cpp:
+ * 1474-1488); `index` the initial slot->view table (NULL = identity). Per-view
+ * sin/cos are evaluated once on the host in double and rounded to float.
+ * Replaces the geometry upload of main() (Tomo_OSTR.cu:2653-2690). */
+int ostr_create(const ostr_geom* geom, const float* angles_rad, const int32_t* index,
+                int32_t n_views, ostr_handle** out);
+void ostr_destroy(ostr_handle* h);
+/* Message of the last failure on h (h == NULL: last failure of ostr_create or
+ * of a call given a NULL handle, per thread). Never NULL. */
+const char* ostr_last_error(const ostr_handle* h);
+/* Replace the slot->view table (OS regroup, Tomo_OSTR_gold.cpp:628-642).
+ * Synchronous. */
+int ostr_set_index(ostr_handle* h, const int32_t* index);
+int ostr_get_geom(const ostr_handle* h, ostr_geom* out);
+int32_t ostr_n_views(const ostr_handle* h);
+
+/* ------------------------------------------- reference launcher mirrors -- */
+
+/* Forward projection of slots [a0,a1): prj[(a-a0)][i][j] = cos_t * sum_j a_ij x_j
+ * (cos_weight != 0) or the plain Siddon line integral (cos_weight == 0).
+ * Replaces fprojectCB_1R_GPU_OSTR_cos (Tomo_OSTR.cu:2334-2359; kernel
+ * :1313-1376, walker :612-890). Difference: rays that miss the volume are
+ * written as 0 instead of left untouched (the reference driver zeroes the
+ * buffer before every call, Tomo_OSTR_gold.cpp:843, so the result is equal). */
+int ostr_fproj(ostr_handle* h, const void* img, int32_t img_dtype, void* prj,
+               int32_t prj_dtype, int32_t a0, int32_t a1, int32_t cos_weight, void* stream);
+
+/* Denominator row term gamma for slots [a0,a1) into gamma[(a-a0)][i][j] (f32).
+ * Replaces fprojectCB_1R_GPU_OSTR_normprj (Tomo_OSTR.cu:2361-2393, walker
+ * :892-1173). mode = OSTR_GAMMA_*. Misses are written as 0. */
+int ostr_gamma(ostr_handle* h, float* gamma, int32_t a0, int32_t a1, int32_t mode,
+               void* stream);
+
+/* Voxel-driven bilinear backprojection of slots [a0,a1), ACCUMULATED into img
+ * (f32). Replaces bprojectCB_GPU_SBP (Tomo_OSTR.cu:2435-2457, kernel
+ * :1553-1734). prj is [(a-a0)][nbx][nby] in prj_dtype. */
+int ostr_bproj_sbp(ostr_handle* h, float* img_accum, const void* prj, int32_t prj_dtype,
+                   int32_t a0, int32_t a1, void* stream);
+
+/* Prior gradient and curvature ACCUMULATED into rd and rdd (f32):
+ * rd_j += sum_k w_jk psi'(x_j - x_k), rdd_j += sum_k w_jk omega(x_j - x_k) over
+ * the 26 neighbours, w = 1/sqrt(di^2+dj^2+(z_xy_ratio dk)^2), out-of-volume
+ * neighbours replaced by the voxel itself. Replaces prior_GPU_OSTR
+ * (Tomo_OSTR.cu:2216-2245) and prior_GPU_OSTR_Q2 (:2189-2212). delta must be
+ * non-zero for LANGE and HYPERBOLA (the reference exits, :2232-2236). */
+int ostr_prior(ostr_handle* h, float* rdd, float* rd, const void* img, int32_t img_dtype,
+               int32_t potential, float delta, float z_xy_ratio, void* stream);
+
+/* -------------------------------------- host CPU ops, now on the device -- */
+
+/* yry[i] = (y-r)^2/y with y-r <= 0 -> y, and 0 where y == 0
+ * (compute_yry, Tomo_OSTR_gold.cpp:183-205). r may be NULL (r = 0). */
+int ostr_yry(ostr_handle* h, float* yry, const float* y, const float* r, int64_t n,
+             void* stream);
+/* out[i] = a[i] * b[i] (compute_gamma_yry, Tomo_OSTR_gold.cpp:208-218). */
+int ostr_mul(ostr_handle* h, float* out, const float* a, const float* b, int64_t n,
+             void* stream);
+/* hsub[i] = (y/(b e^{-l} + r) - 1) b e^{-l} (compute_h, Tomo_OSTR_gold.cpp:
+ * 245-260). r may be NULL. y/b/r/h in f32. */
+int ostr_compute_h(ostr_handle* h, float* hsub, const float* y, const float* b,
+                   const float* l, const float* r, int64_t n, void* stream);
+/* est = max(0, est - (capl_scale*capL + beta*rd)/(d + 2 beta rdd)) over the
+ * volume (capL scaling Tomo_OSTR_gold.cpp:767-771 + update_est :265-295).
+ * guard != 0: voxels with a zero denominator are left unchanged (the reference
+ * produces 0/0 = NaN there, guard == 0 reproduces that). */
+int ostr_update_est(ostr_handle* h, float* est, const float* capL, const float* rd,
+                    const float* d, const float* rdd, float beta, float capl_scale,
+                    int32_t guard, void* stream);
+/* Subset-major regroup of a [n_views][nbx*nby] f32 array: dst[m][j] =
+ * src[j*M + m] (regroup_prj, Tomo_OSTR_gold.cpp:220-242). dst != src. */
+int ostr_regroup(ostr_handle* h, float* dst, const float* src, int32_t n_subsets,
+                 void* stream);
+
+/* ---------------------------------------------------------- fused path -- */
+
+/* d = SBP_all(gamma * yry) over all n_views slots, written (not accumulated)
+ * to d (f32). Fuses ostr_gamma + ostr_yry + ostr_mul + ostr_bproj_sbp of the
+ * reference precompute (Tomo_OSTR_gold.cpp:409-560). y, r in natural view
+ * order [n_views][nbx][nby] f32 (r may be NULL). Uses a device work buffer of
+ * n_views*nbx*nby floats allocated on the first call and kept by the handle. */
+int ostr_precompute_d(ostr_handle* h, float* d, const float* y, const float* r,
+                      int32_t gamma_mode, void* stream);
+
+/* Forward projection of slots [a0,a1) with the h epilogue: hsub = h(y, b, r,
+ * cos_t * A x). y, b, r are the subset's slices [(a-a0)][nbx][nby] (r may be
+ * NULL); hsub in prj_dtype; y/b/r in f32. Fuses ostr_fproj + ostr_compute_h. */
+int ostr_fproj_h(ostr_handle* h, const void* img, int32_t img_dtype, const float* y,
+                 const float* b, const float* r, void* hsub, int32_t prj_dtype,
+                 int32_t a0, int32_t a1, void* stream);
+
+/* x_out = max(0, x_in - (M*SBP(hsub) + beta*RD(x_in)) / (d + 2 beta RDD(x_in)))
+ * in one pass: SBP gather of the subset's h, 27-neighbour prior on x_in
+ * (Jacobi: the prior sees the pre-update image, Tomo_OSTR_gold.cpp:775 before
+ * :814) and the update. x_in and x_out must not alias. */
+int ostr_update_fused(ostr_handle* h, const void* x_in, void* x_out, int32_t img_dtype,
+                      const float* d, const void* hsub, int32_t prj_dtype, int32_t a0,
+                      int32_t a1, float beta, float capl_scale, int32_t potential,
+                      float delta, float z_xy_ratio, int32_t guard, void* stream);
+
+/* ------------------------------------------------------- measurement -- */
+
+/* Number of (ray, voxel) pairs with a_ij > 0 for slots [a0,a1): the exact nnz
+ * of the subset's system matrix. Synchronous; result written to *nnz_out. */
+int ostr_count_nnz(ostr_handle* h, int32_t a0, int32_t a1, int64_t* nnz_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OSTR_HIP_H */

@@ -1,0 +1,446 @@
+// capi.hip - the extern "C" surface of libostr_hip.so (include/ostr_hip.h):
+// handle lifetime, argument validation, error reporting and dispatch to the
+// kernel launchers. No process-global geometry (the reference keeps it in
+// __constant__ symbols, Tomo_OSTR.cu:81-115) and no exit(): every failure is
+// an error code plus a message.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ostr_internal.h"
+
+#define OSTR_API extern "C" __attribute__((visibility("default")))
+
+struct ostr_handle {
+    ostr_geom geom;
+    ostr::DevGeom dg;
+    int n_views = 0;
+    int device = 0;
+    std::vector<float> angles;
+    std::vector<int32_t> index;
+    std::vector<ostr::ViewTab> views_host;  // current slot order
+    std::vector<ostr::ViewTab> views_nat;   // identity order
+    ostr::ViewTab* views_dev = nullptr;
+    ostr::ViewTab* views_nat_dev = nullptr;
+    unsigned long long* counter = nullptr;
+    float* work = nullptr;
+    size_t work_elems = 0;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+int fail(ostr_handle* h, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h)
+        h->err = buf;
+    else
+        g_err = buf;
+    return code;
+}
+
+int hip_fail(ostr_handle* h, hipError_t e, const char* where) {
+    return fail(h, OSTR_E_HIP, "%s: HIP error %d (%s)", where, (int)e, hipGetErrorString(e));
+}
+
+#define CHECK_HANDLE(h)                                                    \
+    do {                                                                   \
+        if (!(h)) return fail(nullptr, OSTR_E_ARG, "%s: NULL handle", __func__); \
+    } while (0)
+
+int use_device(ostr_handle* h) {
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+    return OSTR_OK;
+}
+
+int check_range(ostr_handle* h, int a0, int a1, const char* fn) {
+    if (a0 < 0 || a1 > h->n_views || a0 > a1)
+        return fail(h, OSTR_E_RANGE, "%s: view range [%d,%d) outside [0,%d)", fn, a0, a1,
+                    h->n_views);
+    return OSTR_OK;
+}
+
+int check_dtype(ostr_handle* h, int dt, const char* fn) {
+    if (dt != OSTR_F32 && dt != OSTR_BF16 && dt != OSTR_F16)
+        return fail(h, OSTR_E_UNSUPPORTED, "%s: unknown dtype %d", fn, dt);
+    return OSTR_OK;
+}
+
+ostr::ViewTab view_of(const ostr_geom& g, float theta) {
+    ostr::ViewTab v;
+    v.s = (float)std::sin((double)theta);
+    v.c = (float)std::cos((double)theta);
+    v.x0 = g.source_r * v.s;
+    v.z0 = g.source_r * v.c;
+    return v;
+}
+
+int upload_views(ostr_handle* h) {
+    h->views_host.resize(h->n_views);
+    for (int a = 0; a < h->n_views; ++a)
+        h->views_host[a] = view_of(h->geom, h->angles[h->index[a]]);
+    hipError_t e = hipMemcpy(h->views_dev, h->views_host.data(),
+                             sizeof(ostr::ViewTab) * h->n_views, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(h, e, "upload view table");
+    return OSTR_OK;
+}
+
+int validate_geom(const ostr_geom* g) {
+    if (!g) return fail(nullptr, OSTR_E_ARG, "ostr_create: NULL geometry");
+    if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0 || g->nbx <= 0 || g->nby <= 0)
+        return fail(nullptr, OSTR_E_ARG, "ostr_create: grid sizes must be positive");
+    if (g->nx >= 32768 || g->ny >= 32768 || g->nz >= 32768)
+        return fail(nullptr, OSTR_E_ARG, "ostr_create: volume dimensions must be < 32768");
+    if (!(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0) || !(g->dbx > 0) || !(g->dby > 0))
+        return fail(nullptr, OSTR_E_ARG, "ostr_create: voxel and bin sizes must be positive");
+    if ((int64_t)g->nbx * g->nby > (int64_t)1 << 30)
+        return fail(nullptr, OSTR_E_ARG, "ostr_create: detector too large");
+    return OSTR_OK;
+}
+
+}  // namespace
+
+namespace ostr {
+
+DevGeom make_dev_geom(const ostr_geom& g) {
+    DevGeom d;
+    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz; d.nbx = g.nbx; d.nby = g.nby;
+    d.dx = g.dx; d.dy = g.dy; d.dz = g.dz;
+    d.x_p0 = g.x_p0; d.y_p0 = g.y_p0; d.z_p0 = g.z_p0;
+    d.dbx = g.dbx; d.dby = g.dby; d.x_d0 = g.x_d0; d.y_d0 = g.y_d0;
+    d.det_r = g.detector_r; d.src_r = g.source_r; d.src_y = g.source_y;
+    d.ext_x = (float)g.nx * g.dx;
+    d.ext_y = (float)g.ny * g.dy;
+    d.ext_z = (float)g.nz * g.dz;
+    d.top_x = std::fma((float)g.nx, g.dx, g.x_p0);
+    d.top_y = std::fma((float)g.ny, g.dy, g.y_p0);
+    d.top_z = std::fma((float)g.nz, g.dz, g.z_p0);
+    d.hp_x = g.dx + g.x_p0;
+    d.hp_y = g.dy + g.y_p0;
+    d.hp_z = g.dz + g.z_p0;
+    d.last_x = std::fma((float)(g.nx - 1), g.dx, g.x_p0);
+    d.last_y = std::fma((float)(g.ny - 1), g.dy, g.y_p0);
+    d.last_z = std::fma((float)(g.nz - 1), g.dz, g.z_p0);
+    return d;
+}
+
+}  // namespace ostr
+
+// ======================================================================
+OSTR_API int ostr_create(const ostr_geom* geom, const float* angles_rad, const int32_t* index,
+                         int32_t n_views, ostr_handle** out) {
+    if (!out) return fail(nullptr, OSTR_E_ARG, "ostr_create: NULL output pointer");
+    *out = nullptr;
+    int rc = validate_geom(geom);
+    if (rc) return rc;
+    if (n_views <= 0 || !angles_rad)
+        return fail(nullptr, OSTR_E_ARG, "ostr_create: need n_views > 0 and angles");
+    for (int i = 0; i < n_views; ++i)
+        if (!std::isfinite(angles_rad[i]))
+            return fail(nullptr, OSTR_E_ARG, "ostr_create: angle %d is not finite", i);
+    if (index)
+        for (int i = 0; i < n_views; ++i)
+            if (index[i] < 0 || index[i] >= n_views)
+                return fail(nullptr, OSTR_E_ARG, "ostr_create: index[%d]=%d out of range", i,
+                            index[i]);
+    ostr_handle* h = new ostr_handle();
+    h->geom = *geom;
+    h->dg = ostr::make_dev_geom(*geom);
+    h->n_views = n_views;
+    h->angles.assign(angles_rad, angles_rad + n_views);
+    h->index.resize(n_views);
+    for (int i = 0; i < n_views; ++i) h->index[i] = index ? index[i] : i;
+    hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess) e = hipMalloc(&h->views_dev, sizeof(ostr::ViewTab) * n_views);
+    if (e == hipSuccess) e = hipMalloc(&h->views_nat_dev, sizeof(ostr::ViewTab) * n_views);
+    if (e == hipSuccess) e = hipMalloc(&h->counter, sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        hip_fail(nullptr, e, "ostr_create");
+        ostr_destroy(h);
+        return OSTR_E_HIP;
+    }
+    h->views_nat.resize(n_views);
+    for (int a = 0; a < n_views; ++a) h->views_nat[a] = view_of(*geom, h->angles[a]);
+    e = hipMemcpy(h->views_nat_dev, h->views_nat.data(), sizeof(ostr::ViewTab) * n_views,
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess || upload_views(h) != OSTR_OK) {
+        if (e != hipSuccess) hip_fail(nullptr, e, "ostr_create");
+        else g_err = h->err;
+        ostr_destroy(h);
+        return OSTR_E_HIP;
+    }
+    *out = h;
+    return OSTR_OK;
+}
+
+OSTR_API void ostr_destroy(ostr_handle* h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->views_dev) (void)hipFree(h->views_dev);
+    if (h->views_nat_dev) (void)hipFree(h->views_nat_dev);
+    if (h->counter) (void)hipFree(h->counter);
+    if (h->work) (void)hipFree(h->work);
+    delete h;
+}
+
+OSTR_API const char* ostr_last_error(const ostr_handle* h) {
+    return h ? h->err.c_str() : g_err.c_str();
+}
+
+OSTR_API int ostr_set_index(ostr_handle* h, const int32_t* index) {
+    CHECK_HANDLE(h);
+    if (!index) return fail(h, OSTR_E_ARG, "ostr_set_index: NULL index");
+    for (int i = 0; i < h->n_views; ++i)
+        if (index[i] < 0 || index[i] >= h->n_views)
+            return fail(h, OSTR_E_ARG, "ostr_set_index: index[%d]=%d out of range", i, index[i]);
+    int rc = use_device(h);
+    if (rc) return rc;
+    // the view table may still be read by queued kernels on any stream
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(h, e, "ostr_set_index");
+    h->index.assign(index, index + h->n_views);
+    return upload_views(h);
+}
+
+OSTR_API int ostr_get_geom(const ostr_handle* h, ostr_geom* out) {
+    if (!h || !out) return fail(nullptr, OSTR_E_ARG, "ostr_get_geom: NULL argument");
+    *out = h->geom;
+    return OSTR_OK;
+}
+
+OSTR_API int32_t ostr_n_views(const ostr_handle* h) { return h ? h->n_views : -1; }
+
+// ------------------------------------------------------------ launchers --
+OSTR_API int ostr_fproj(ostr_handle* h, const void* img, int32_t img_dtype, void* prj,
+                        int32_t prj_dtype, int32_t a0, int32_t a1, int32_t cos_weight,
+                        void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_fproj")) || (rc = check_dtype(h, img_dtype, "ostr_fproj")) ||
+        (rc = check_dtype(h, prj_dtype, "ostr_fproj")))
+        return rc;
+    if (!img || !prj) return fail(h, OSTR_E_ARG, "ostr_fproj: NULL buffer");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_project(h->dg, h->views_dev, a0, a1, 0, cos_weight, img,
+                                        img_dtype, prj, prj_dtype, nullptr, nullptr, nullptr,
+                                        (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_fproj");
+}
+
+OSTR_API int ostr_gamma(ostr_handle* h, float* gamma, int32_t a0, int32_t a1, int32_t mode,
+                        void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_gamma"))) return rc;
+    if (mode != OSTR_GAMMA_RAY_LENGTH && mode != OSTR_GAMMA_ROW_SUM)
+        return fail(h, OSTR_E_ARG, "ostr_gamma: unknown mode %d", mode);
+    if (!gamma) return fail(h, OSTR_E_ARG, "ostr_gamma: NULL buffer");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_project(h->dg, h->views_dev, a0, a1,
+                                        mode == OSTR_GAMMA_ROW_SUM ? 2 : 1, 0, nullptr,
+                                        OSTR_F32, gamma, OSTR_F32, nullptr, nullptr, nullptr,
+                                        (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_gamma");
+}
+
+OSTR_API int ostr_bproj_sbp(ostr_handle* h, float* img_accum, const void* prj,
+                            int32_t prj_dtype, int32_t a0, int32_t a1, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_bproj_sbp")) ||
+        (rc = check_dtype(h, prj_dtype, "ostr_bproj_sbp")))
+        return rc;
+    if (!img_accum || !prj) return fail(h, OSTR_E_ARG, "ostr_bproj_sbp: NULL buffer");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_sbp(h->dg, h->views_host.data(), a0, a1, img_accum, prj,
+                                    prj_dtype, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_bproj_sbp");
+}
+
+OSTR_API int ostr_prior(ostr_handle* h, float* rdd, float* rd, const void* img,
+                        int32_t img_dtype, int32_t potential, float delta, float z_xy_ratio,
+                        void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_dtype(h, img_dtype, "ostr_prior"))) return rc;
+    if (potential < OSTR_PRIOR_Q2 || potential > OSTR_PRIOR_HYPERBOLA)
+        return fail(h, OSTR_E_ARG, "ostr_prior: unknown potential %d", potential);
+    if (potential != OSTR_PRIOR_Q2 && delta == 0.0f)
+        return fail(h, OSTR_E_ARG, "ostr_prior: delta cannot be zero");
+    if (!(z_xy_ratio > 0.0f)) return fail(h, OSTR_E_ARG, "ostr_prior: z_xy_ratio must be > 0");
+    if (!rdd || !rd || !img) return fail(h, OSTR_E_ARG, "ostr_prior: NULL buffer");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_prior(h->dg, rdd, rd, img, img_dtype, potential, delta,
+                                      z_xy_ratio, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_prior");
+}
+
+// --------------------------------------------------------- elementwise --
+OSTR_API int ostr_yry(ostr_handle* h, float* yry, const float* y, const float* r, int64_t n,
+                      void* stream) {
+    CHECK_HANDLE(h);
+    if (!yry || !y || n < 0) return fail(h, OSTR_E_ARG, "ostr_yry: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_yry(yry, y, r, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_yry");
+}
+
+OSTR_API int ostr_mul(ostr_handle* h, float* out, const float* a, const float* b, int64_t n,
+                      void* stream) {
+    CHECK_HANDLE(h);
+    if (!out || !a || !b || n < 0) return fail(h, OSTR_E_ARG, "ostr_mul: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_mul(out, a, b, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_mul");
+}
+
+OSTR_API int ostr_compute_h(ostr_handle* h, float* hsub, const float* y, const float* b,
+                            const float* l, const float* r, int64_t n, void* stream) {
+    CHECK_HANDLE(h);
+    if (!hsub || !y || !b || !l || n < 0)
+        return fail(h, OSTR_E_ARG, "ostr_compute_h: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_compute_h(hsub, y, b, l, r, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_compute_h");
+}
+
+OSTR_API int ostr_update_est(ostr_handle* h, float* est, const float* capL, const float* rd,
+                             const float* d, const float* rdd, float beta, float capl_scale,
+                             int32_t guard, void* stream) {
+    CHECK_HANDLE(h);
+    if (!est || !capL || !rd || !d || !rdd)
+        return fail(h, OSTR_E_ARG, "ostr_update_est: NULL buffer");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    int64_t n = (int64_t)h->geom.nx * h->geom.ny * h->geom.nz;
+    hipError_t e = ostr::launch_update_est(est, capL, rd, d, rdd, beta, capl_scale, guard, n,
+                                           (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_update_est");
+}
+
+OSTR_API int ostr_regroup(ostr_handle* h, float* dst, const float* src, int32_t n_subsets,
+                          void* stream) {
+    CHECK_HANDLE(h);
+    if (!dst || !src || dst == src) return fail(h, OSTR_E_ARG, "ostr_regroup: bad buffers");
+    if (n_subsets <= 0 || h->n_views % n_subsets)
+        return fail(h, OSTR_E_ARG, "ostr_regroup: %d views not divisible into %d subsets",
+                    h->n_views, n_subsets);
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_regroup(dst, src, h->n_views, n_subsets,
+                                        (int64_t)h->geom.nbx * h->geom.nby, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_regroup");
+}
+
+// ---------------------------------------------------------------- fused --
+OSTR_API int ostr_precompute_d(ostr_handle* h, float* d, const float* y, const float* r,
+                               int32_t gamma_mode, void* stream) {
+    CHECK_HANDLE(h);
+    if (!d || !y) return fail(h, OSTR_E_ARG, "ostr_precompute_d: NULL buffer");
+    if (gamma_mode != OSTR_GAMMA_RAY_LENGTH && gamma_mode != OSTR_GAMMA_ROW_SUM)
+        return fail(h, OSTR_E_ARG, "ostr_precompute_d: unknown gamma mode %d", gamma_mode);
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nall = (int64_t)h->n_views * h->geom.nbx * h->geom.nby;
+    if (h->work_elems < (size_t)nall) {
+        if (h->work) {
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(h->work);
+            h->work = nullptr;
+            h->work_elems = 0;
+        }
+        hipError_t e = hipMalloc(&h->work, sizeof(float) * nall);
+        if (e != hipSuccess) return hip_fail(h, e, "ostr_precompute_d: work buffer");
+        h->work_elems = nall;
+    }
+    const int64_t nvox = (int64_t)h->geom.nx * h->geom.ny * h->geom.nz;
+    // gamma and d use the natural view order (computed before the OS regroup,
+    // Tomo_OSTR_gold.cpp:450-538)
+    hipError_t e = ostr::launch_project(h->dg, h->views_nat_dev, 0, h->n_views,
+                                        gamma_mode == OSTR_GAMMA_ROW_SUM ? 2 : 1, 0, nullptr,
+                                        OSTR_F32, h->work, OSTR_F32, nullptr, nullptr, nullptr,
+                                        st);
+    if (e == hipSuccess) e = ostr::launch_yry_gamma(h->work, y, r, nall, st);
+    if (e == hipSuccess) e = ostr::launch_fill(d, 0.0f, nvox, st);
+    if (e == hipSuccess)
+        e = ostr::launch_sbp(h->dg, h->views_nat.data(), 0, h->n_views, d, h->work, OSTR_F32,
+                             st);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_precompute_d");
+}
+
+OSTR_API int ostr_fproj_h(ostr_handle* h, const void* img, int32_t img_dtype, const float* y,
+                          const float* b, const float* r, void* hsub, int32_t prj_dtype,
+                          int32_t a0, int32_t a1, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_fproj_h")) ||
+        (rc = check_dtype(h, img_dtype, "ostr_fproj_h")) ||
+        (rc = check_dtype(h, prj_dtype, "ostr_fproj_h")))
+        return rc;
+    if (!img || !y || !b || !hsub) return fail(h, OSTR_E_ARG, "ostr_fproj_h: NULL buffer");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_project(h->dg, h->views_dev, a0, a1, 3, 1, img, img_dtype, hsub,
+                                        prj_dtype, y, b, r, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_fproj_h");
+}
+
+OSTR_API int ostr_update_fused(ostr_handle* h, const void* x_in, void* x_out, int32_t img_dtype,
+                               const float* d, const void* hsub, int32_t prj_dtype, int32_t a0,
+                               int32_t a1, float beta, float capl_scale, int32_t potential,
+                               float delta, float z_xy_ratio, int32_t guard, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_update_fused")) ||
+        (rc = check_dtype(h, img_dtype, "ostr_update_fused")) ||
+        (rc = check_dtype(h, prj_dtype, "ostr_update_fused")))
+        return rc;
+    if (!x_in || !x_out || !d || !hsub)
+        return fail(h, OSTR_E_ARG, "ostr_update_fused: NULL buffer");
+    if (x_in == x_out) return fail(h, OSTR_E_ARG, "ostr_update_fused: x_in and x_out alias");
+    if (potential < OSTR_PRIOR_Q2 || potential > OSTR_PRIOR_HYPERBOLA)
+        return fail(h, OSTR_E_ARG, "ostr_update_fused: unknown potential %d", potential);
+    if (potential != OSTR_PRIOR_Q2 && delta == 0.0f)
+        return fail(h, OSTR_E_ARG, "ostr_update_fused: delta cannot be zero");
+    if (!(z_xy_ratio > 0.0f))
+        return fail(h, OSTR_E_ARG, "ostr_update_fused: z_xy_ratio must be > 0");
+    if (a1 - a0 > 64) return fail(h, OSTR_E_RANGE, "ostr_update_fused: at most 64 views per subset");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_update_fused(h->dg, h->views_host.data(), a0, a1, x_in, x_out,
+                                             img_dtype, d, hsub, prj_dtype, beta, capl_scale,
+                                             potential, delta, z_xy_ratio, guard,
+                                             (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_update_fused");
+}
+
+OSTR_API int ostr_count_nnz(ostr_handle* h, int32_t a0, int32_t a1, int64_t* nnz_out,
+                            void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_count_nnz"))) return rc;
+    if (!nnz_out) return fail(h, OSTR_E_ARG, "ostr_count_nnz: NULL output");
+    if ((rc = use_device(h))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned long long host = 0;
+    hipError_t e = hipMemcpyAsync(h->counter, &host, sizeof host, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = ostr::launch_count_nnz(h->dg, h->views_dev, a0, a1, h->counter, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&host, h->counter, sizeof host, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(h, e, "ostr_count_nnz");
+    *nnz_out = (int64_t)host;
+    return OSTR_OK;
+}

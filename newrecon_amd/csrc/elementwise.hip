@@ -1,0 +1,151 @@
+// elementwise.hip - the reference's host-side CPU loops, as device kernels:
+//   compute_yry        Tomo_OSTR_gold.cpp:183-205
+//   compute_gamma_yry  Tomo_OSTR_gold.cpp:208-218
+//   regroup_prj        Tomo_OSTR_gold.cpp:220-242
+//   compute_h          Tomo_OSTR_gold.cpp:245-260
+//   capL * M + update_est  Tomo_OSTR_gold.cpp:767-771, 265-295
+// Grid-stride loops, 256 threads, 4 elements in flight per lane.
+#include "ostr_internal.h"
+
+namespace ostr {
+
+namespace {
+
+inline unsigned grid_for(int64_t n) {
+    int64_t blocks = (n + 1023) / 1024;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 65536) blocks = 65536;
+    return (unsigned)blocks;
+}
+
+__device__ __forceinline__ float yry_one(float y, float r) {
+    if (y == 0.0f) return 0.0f;
+    float dif = y - r;
+    if (dif <= 0.0f) dif = y;
+    return (dif * dif) / y;
+}
+
+__global__ __launch_bounds__(256) void yry_kernel(float* __restrict__ out,
+                                                  const float* __restrict__ y,
+                                                  const float* __restrict__ r, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        out[i] = yry_one(y[i], r ? r[i] : 0.0f);
+}
+
+// gc = gamma * yry with gamma already in gc (fused precompute)
+__global__ __launch_bounds__(256) void yry_gamma_kernel(float* __restrict__ gc,
+                                                        const float* __restrict__ y,
+                                                        const float* __restrict__ r, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        gc[i] = yry_one(y[i], r ? r[i] : 0.0f) * gc[i];
+}
+
+__global__ __launch_bounds__(256) void mul_kernel(float* __restrict__ out,
+                                                  const float* __restrict__ a,
+                                                  const float* __restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        out[i] = a[i] * b[i];
+}
+
+__global__ __launch_bounds__(256) void h_kernel(float* __restrict__ h,
+                                                const float* __restrict__ y,
+                                                const float* __restrict__ b,
+                                                const float* __restrict__ l,
+                                                const float* __restrict__ r, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
+        float yt = b[i] * expf(-l[i]);
+        float den = r ? yt + r[i] : yt;
+        h[i] = (y[i] / den - 1.0f) * yt;
+    }
+}
+
+__global__ __launch_bounds__(256) void update_kernel(float* __restrict__ x,
+                                                     const float* __restrict__ capl,
+                                                     const float* __restrict__ rd,
+                                                     const float* __restrict__ d,
+                                                     const float* __restrict__ rdd, float beta,
+                                                     float capl_scale, int guard, int64_t n) {
+    const float two_beta = 2.0f * beta;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
+        float num = capl_scale * capl[i] + beta * rd[i];
+        float den = d[i] + two_beta * rdd[i];
+        if (guard && den == 0.0f) continue;
+        float v = x[i] - num / den;
+        x[i] = v < 0.0f ? 0.0f : v;
+    }
+}
+
+// dst block k = m*vs + j  <-  src block j*M + m, blocks of nbin floats
+__global__ __launch_bounds__(256) void regroup_kernel(float* __restrict__ dst,
+                                                      const float* __restrict__ src,
+                                                      int n_views, int n_subsets, int64_t nbin) {
+    const int vs = n_views / n_subsets;
+    const int k = blockIdx.y;
+    const int m = k / vs, j = k % vs;
+    const float* s = src + (int64_t)(j * n_subsets + m) * nbin;
+    float* o = dst + (int64_t)k * nbin;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nbin;
+         i += (int64_t)gridDim.x * 256)
+        o[i] = s[i];
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, float v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        p[i] = v;
+}
+
+}  // namespace
+
+hipError_t launch_fill(float* p, float v, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_yry(float* out, const float* y, const float* r, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(yry_kernel, dim3(grid_for(n)), dim3(256), 0, st, out, y, r, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_yry_gamma(float* gc, const float* y, const float* r, int64_t n,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(yry_gamma_kernel, dim3(grid_for(n)), dim3(256), 0, st, gc, y, r, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_mul(float* out, const float* a, const float* b, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(mul_kernel, dim3(grid_for(n)), dim3(256), 0, st, out, a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_compute_h(float* h, const float* y, const float* b, const float* l,
+                            const float* r, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(h_kernel, dim3(grid_for(n)), dim3(256), 0, st, h, y, b, l, r, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_est(float* x, const float* capl, const float* rd, const float* d,
+                             const float* rdd, float beta, float capl_scale, int guard,
+                             int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(update_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, capl, rd, d, rdd,
+                       beta, capl_scale, guard, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_regroup(float* dst, const float* src, int n_views, int n_subsets,
+                          int64_t nbin, hipStream_t st) {
+    int64_t bx = (nbin + 1023) / 1024;
+    if (bx > 4096) bx = 4096;
+    if (bx < 1) bx = 1;
+    hipLaunchKernelGGL(regroup_kernel, dim3((unsigned)bx, n_views), dim3(256), 0, st, dst, src,
+                       n_views, n_subsets, nbin);
+    return hipGetLastError();
+}
+
+}  // namespace ostr

@@ -1,0 +1,82 @@
+// prior.hip - standalone prior gradient/curvature kernel (accumulating),
+// the counterpart of prior_GPU_OSTR / prior_GPU_OSTR_Q2 (Tomo_OSTR.cu:
+// 2189-2245; kernels G_Q2_prior :118-184, G_Fessler_prior :187-255, and the
+// hyperbola potential of G_Huber_prior :258-325).
+//
+// One lane per voxel, 64 lanes along x; neighbour reads go through L1/L2.
+#include <cmath>
+
+#include "prior_device.h"
+
+namespace ostr {
+
+void prior_weights(float w[27], float rho) {
+    int n = 0;
+    for (int dk = -1; dk <= 1; ++dk)
+        for (int dj = -1; dj <= 1; ++dj)
+            for (int di = -1; di <= 1; ++di) {
+                float dist = std::sqrt((float)(di * di + dj * dj) + ((float)(dk * dk) * rho) * rho);
+                w[n++] = dist == 0.0f ? 0.0f : (float)(1.0 / (double)dist);
+            }
+}
+
+namespace {
+
+template <int POT, typename TI>
+__global__ __launch_bounds__(256) void prior_kernel(DevGeom g, PriorW W, float inv_delta,
+                                                    const TI* __restrict__ x,
+                                                    float* __restrict__ rdd,
+                                                    float* __restrict__ rd) {
+    const int ix = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int iz = blockIdx.z;
+    if (ix >= g.nx || iy >= g.ny) return;
+    const int64_t vi = ((int64_t)iz * g.ny + iy) * g.nx + ix;
+    float a = rdd[vi], b = rd[vi];
+    prior_voxel<POT, TI>(g, x, ix, iy, iz, W, inv_delta, a, b);
+    rdd[vi] = a;
+    rd[vi] = b;
+}
+
+template <typename TI>
+hipError_t launch_t(const DevGeom& g, float* rdd, float* rd, const void* img, int pot,
+                    const PriorW& W, float inv_delta, hipStream_t st) {
+    dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+    switch (pot) {
+        case OSTR_PRIOR_Q2:
+            hipLaunchKernelGGL((prior_kernel<OSTR_PRIOR_Q2, TI>), grid, dim3(256), 0, st, g, W,
+                               inv_delta, (const TI*)img, rdd, rd);
+            break;
+        case OSTR_PRIOR_LANGE:
+            hipLaunchKernelGGL((prior_kernel<OSTR_PRIOR_LANGE, TI>), grid, dim3(256), 0, st, g,
+                               W, inv_delta, (const TI*)img, rdd, rd);
+            break;
+        case OSTR_PRIOR_HYPERBOLA:
+            hipLaunchKernelGGL((prior_kernel<OSTR_PRIOR_HYPERBOLA, TI>), grid, dim3(256), 0, st,
+                               g, W, inv_delta, (const TI*)img, rdd, rd);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_prior(const DevGeom& g, float* rdd, float* rd, const void* img,
+                        int img_dtype, int potential, float delta, float rho, hipStream_t st) {
+    PriorW W;
+    prior_weights(W.w, rho);
+    float inv_delta = potential == OSTR_PRIOR_Q2 ? 0.0f : 1.0f / delta;
+    switch (img_dtype) {
+        case OSTR_F32:
+            return launch_t<float>(g, rdd, rd, img, potential, W, inv_delta, st);
+        case OSTR_BF16:
+            return launch_t<bf16_t>(g, rdd, rd, img, potential, W, inv_delta, st);
+        case OSTR_F16:
+            return launch_t<_Float16>(g, rdd, rd, img, potential, W, inv_delta, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ostr

@@ -1,0 +1,337 @@
+// siddon.hip - ray-driven exact Siddon forward projector family for gfx950.
+//
+// Semantics follow the reference walker rayTrace3D_GPU_direct_notexturememory_OSTR
+// (Tomo_OSTR.cu:612-890) and the kernels ..._OSTR_cos / ..._normprj
+// (Tomo_OSTR.cu:1313-1449): slab test, entry voxel, incremental plane
+// crossings with exact tie handling, relative stop test and tail segment, the
+// same fma placement (expressions `a*b + c` of the reference, which nvcc
+// contracts) and no other contraction (built with -ffp-contract=off).
+//
+// MI355X mapping: one ray per lane, 64 lanes along detector x (the direction
+// that is contiguous in the volume, so gathers of a wave fall in a few cache
+// lines); a work-group is a 64 x 4 bin tile of one view slot; tiles are dealt
+// to XCDs in contiguous ranges so the frustum a range reads stays in one L2.
+// Per-view trig comes from the handle's table (host FP64, rounded to float).
+// The relative stop test (lmax - at)/lmax > 1e-6 is replaced by `at < thr`
+// with thr computed once per ray as the exact float boundary of that test, so
+// the loop has no division and takes the same steps.
+#include "ostr_internal.h"
+
+namespace ostr {
+
+namespace {
+
+constexpr int TILE_I = 64;   // bins along detector x per tile (one per lane)
+constexpr int TILE_J = 4;    // bins along detector y per tile (one per wave)
+
+__device__ __forceinline__ float bin_centre(float origin, int i, float size) {
+    return (float)((double)origin + ((double)i + 0.5) * (double)size);
+}
+
+struct Axis {
+    float lo, hi, next, inc;
+    int p, step;
+};
+
+__device__ __forceinline__ bool axis_limits(Axis& a, float s, float e, float p0, float ext,
+                                            float top) {
+    float d = e - s;
+    if (d != 0.0f) {
+        float lo = (p0 - s) / d;
+        float hi = lo + ext / d;
+        if (lo > hi) {
+            a.lo = hi;
+            a.hi = lo;
+        } else {
+            a.lo = lo;
+            a.hi = hi;
+        }
+        return true;
+    }
+    if (s >= top || s <= p0) return false;
+    a.lo = -1000.0f;
+    a.hi = 1000.0f;
+    return true;
+}
+
+__device__ __forceinline__ void axis_entry(Axis& a, float lmin, float s, float e, float p0,
+                                           float h, int n, float hp, float last) {
+    float d = e - s;
+    if (a.lo != lmin) {
+        a.p = (int)floorf((fmaf(lmin, d, s) - p0) / h);
+        if (s < e)
+            a.next = (fmaf((float)(a.p + 1), h, p0) - s) / d;
+        else if (s == e)
+            a.next = 1000.0f;
+        else
+            a.next = (fmaf((float)a.p, h, p0) - s) / d;
+    } else if (s < e) {
+        a.p = 0;
+        a.next = (hp - s) / d;
+    } else {
+        a.p = n - 1;
+        a.next = (last - s) / d;
+    }
+    float inc = h / d;
+    // next - inc == next + (-inc) exactly, so a signed increment keeps the
+    // reference's two branches in one add.
+    a.inc = (s < e) ? inc : -inc;
+    a.step = (s < e) ? 1 : -1;
+}
+
+__device__ __forceinline__ bool keep_walking(float lmax, float at) {
+    return (lmax - at) / lmax > 1e-6f;
+}
+
+// Smallest float thr with !keep_walking(lmax, thr); for lmax > 0 the test is
+// monotone in `at`, so keep_walking(lmax, at) <=> at < thr.
+__device__ __forceinline__ float stop_threshold(float lmax) {
+    float thr = lmax - lmax * 1e-6f;
+    if (keep_walking(lmax, thr)) {
+        do {
+            thr = __uint_as_float(__float_as_uint(thr) + 1u);
+        } while (keep_walking(lmax, thr));
+    } else {
+        float prev = __uint_as_float(__float_as_uint(thr) - 1u);
+        while (!keep_walking(lmax, prev)) {
+            thr = prev;
+            prev = __uint_as_float(__float_as_uint(thr) - 1u);
+        }
+    }
+    return thr;
+}
+
+// One Siddon ray from the source S to the bin centre E. Returns the weighted
+// sum (ONES: the plain row sum); hit tells whether the ray meets the volume.
+// COUNT: also counts in-volume segments of positive length into *nseg.
+template <bool ONES, bool COUNT, typename TI>
+__device__ __forceinline__ float walk(const DevGeom& g, const TI* __restrict__ img, float sx,
+                                      float sy, float sz, float ex, float ey, float ez,
+                                      bool& hit, float& len_out, int& nseg) {
+    hit = false;
+    float ddx = ex - sx, ddy = ey - sy, ddz = ez - sz;
+    float len = sqrtf(fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)));
+    len_out = len;
+    Axis X, Y, Z;
+    if (!axis_limits(X, sx, ex, g.x_p0, g.ext_x, g.top_x)) return 0.0f;
+    if (!axis_limits(Y, sy, ey, g.y_p0, g.ext_y, g.top_y)) return 0.0f;
+    if (!axis_limits(Z, sz, ez, g.z_p0, g.ext_z, g.top_z)) return 0.0f;
+    float lmax = X.hi;
+    if (lmax > Y.hi) lmax = Y.hi;
+    if (lmax > Z.hi) lmax = Z.hi;
+    float lmin = X.lo;
+    if (lmin < Y.lo) lmin = Y.lo;
+    if (lmin < Z.lo) lmin = Z.lo;
+    if (!(lmin < lmax)) return 0.0f;
+    axis_entry(X, lmin, sx, ex, g.x_p0, g.dx, g.nx, g.hp_x, g.last_x);
+    axis_entry(Y, lmin, sy, ey, g.y_p0, g.dy, g.ny, g.hp_y, g.last_y);
+    axis_entry(Z, lmin, sz, ez, g.z_p0, g.dz, g.nz, g.hp_z, g.last_z);
+
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    int64_t vid = ((int64_t)Z.p * ny + Y.p) * nx + X.p;
+    const int64_t sxs = X.step, sys = (int64_t)Y.step * nx, szs = (int64_t)Z.step * nx * ny;
+    float sum = 0.0f;
+    float cur = lmin;
+    float at = X.next;
+    if (at > Y.next) at = Y.next;
+    if (at > Z.next) at = Z.next;
+
+    const bool fast = lmax > 0.0f;
+    const float thr = fast ? stop_threshold(lmax) : 0.0f;
+    while (fast ? (at < thr) : keep_walking(lmax, at)) {
+        float seg = (at - cur) * len;
+        if ((unsigned)X.p < (unsigned)nx && (unsigned)Y.p < (unsigned)ny &&
+            (unsigned)Z.p < (unsigned)nz) {
+            float v = ONES ? 1.0f : ld<TI>(img, vid);
+            sum = fmaf(v, seg, sum);
+            if (COUNT && seg > 0.0f) ++nseg;
+        }
+        if (at == X.next) {
+            X.p += X.step;
+            X.next = X.next + X.inc;
+            vid += sxs;
+        }
+        if (at == Y.next) {
+            Y.p += Y.step;
+            Y.next = Y.next + Y.inc;
+            vid += sys;
+        }
+        if (at == Z.next) {
+            Z.p += Z.step;
+            Z.next = Z.next + Z.inc;
+            vid += szs;
+        }
+        cur = at;
+        at = X.next;
+        if (at > Y.next) at = Y.next;
+        if (at > Z.next) at = Z.next;
+    }
+    float seg = (lmax - cur) * len;
+    if ((unsigned)X.p < (unsigned)nx && (unsigned)Y.p < (unsigned)ny &&
+        (unsigned)Z.p < (unsigned)nz) {
+        float v = ONES ? 1.0f : ld<TI>(img, vid);
+        sum = fmaf(v, seg, sum);
+        if (COUNT && seg > 0.0f) ++nseg;
+    }
+    hit = true;
+    return sum;
+}
+
+// kind: 0 projection, 1 gamma ray length, 2 gamma row sum, 3 projection + h.
+template <int KIND, typename TI, typename TO>
+__global__ __launch_bounds__(256) void project_kernel(DevGeom g, const ViewTab* __restrict__ views,
+                                                      int a0, int n_slots, int tiles_i,
+                                                      int tiles_j, int cos_weight,
+                                                      const TI* __restrict__ img,
+                                                      TO* __restrict__ out,
+                                                      const float* __restrict__ y,
+                                                      const float* __restrict__ b,
+                                                      const float* __restrict__ r) {
+    const int nblocks = gridDim.x;
+    const int t = xcd_tile(blockIdx.x, nblocks);
+    // tile order: j-tile fastest, then view slot, then i-tile
+    const int tj = t % tiles_j;
+    const int rest = t / tiles_j;
+    const int slot = rest % n_slots;
+    const int ti = rest / n_slots;
+    const int i = ti * TILE_I + (threadIdx.x & 63);
+    const int j = tj * TILE_J + (threadIdx.x >> 6);
+    if (i >= g.nbx || j >= g.nby) return;
+
+    const ViewTab v = views[a0 + slot];
+    const float ex = bin_centre(g.x_d0, i, g.dbx);
+    const float ey = bin_centre(g.y_d0, j, g.dby);
+    const float ez = -g.det_r;
+    bool hit;
+    float len;
+    int nseg = 0;
+    float s;
+    if (KIND == 0 || KIND == 3)
+        s = walk<false, false, TI>(g, img, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
+    else if (KIND == 2)
+        s = walk<true, false, TI>(g, img, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
+    else {
+        walk<true, false, TI>(g, img, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
+        s = len;
+    }
+    float val;
+    if (!hit)
+        val = 0.0f;
+    else if ((KIND == 0 && cos_weight) || KIND == 3)
+        val = s * v.c;
+    else
+        val = s;
+    const int64_t o = ((int64_t)slot * g.nbx + i) * g.nby + j;
+    if (KIND == 3) {
+        // h epilogue (compute_h, Tomo_OSTR_gold.cpp:255-258)
+        float yt = b[o] * expf(-val);
+        float den = r ? yt + r[o] : yt;
+        val = (y[o] / den - 1.0f) * yt;
+    }
+    st<TO>(out, o, val);
+}
+
+__global__ __launch_bounds__(256) void count_kernel(DevGeom g, const ViewTab* __restrict__ views,
+                                                    int a0, int n_slots, int tiles_i,
+                                                    int tiles_j,
+                                                    unsigned long long* __restrict__ counter) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int tj = t % tiles_j;
+    const int rest = t / tiles_j;
+    const int slot = rest % n_slots;
+    const int ti = rest / n_slots;
+    const int i = ti * TILE_I + (threadIdx.x & 63);
+    const int j = tj * TILE_J + (threadIdx.x >> 6);
+    int nseg = 0;
+    if (i < g.nbx && j < g.nby) {
+        const ViewTab v = views[a0 + slot];
+        bool hit;
+        float len;
+        walk<true, true, float>(g, (const float*)nullptr, v.x0, g.src_y, v.z0,
+                                bin_centre(g.x_d0, i, g.dbx), bin_centre(g.y_d0, j, g.dby),
+                                -g.det_r, hit, len, nseg);
+    }
+    // wave reduction, one atomic per wave
+    unsigned long long c = (unsigned long long)nseg;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(counter, c);
+}
+
+template <int KIND, typename TI, typename TO>
+hipError_t launch_kind(const DevGeom& g, const ViewTab* views, int a0, int a1, int cos_weight,
+                       const void* img, void* out, const float* y, const float* b,
+                       const float* r, hipStream_t st) {
+    const int n_slots = a1 - a0;
+    const int tiles_i = (g.nbx + TILE_I - 1) / TILE_I;
+    const int tiles_j = (g.nby + TILE_J - 1) / TILE_J;
+    const int64_t nblocks = (int64_t)tiles_i * tiles_j * n_slots;
+    if (nblocks == 0) return hipSuccess;
+    if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((project_kernel<KIND, TI, TO>), dim3((unsigned)nblocks), dim3(256), 0, st,
+                       g, views, a0, n_slots, tiles_i, tiles_j, cos_weight,
+                       (const TI*)img, (TO*)out, y, b, r);
+    return hipGetLastError();
+}
+
+template <int KIND, typename TI>
+hipError_t launch_out(int out_dtype, const DevGeom& g, const ViewTab* views, int a0, int a1,
+                      int cos_weight, const void* img, void* out, const float* y,
+                      const float* b, const float* r, hipStream_t st) {
+    switch (out_dtype) {
+        case OSTR_F32:
+            return launch_kind<KIND, TI, float>(g, views, a0, a1, cos_weight, img, out, y, b, r, st);
+        case OSTR_BF16:
+            return launch_kind<KIND, TI, bf16_t>(g, views, a0, a1, cos_weight, img, out, y, b, r, st);
+        case OSTR_F16:
+            return launch_kind<KIND, TI, _Float16>(g, views, a0, a1, cos_weight, img, out, y, b, r, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int KIND>
+hipError_t launch_in(int img_dtype, int out_dtype, const DevGeom& g, const ViewTab* views,
+                     int a0, int a1, int cos_weight, const void* img, void* out,
+                     const float* y, const float* b, const float* r, hipStream_t st) {
+    switch (img_dtype) {
+        case OSTR_F32:
+            return launch_out<KIND, float>(out_dtype, g, views, a0, a1, cos_weight, img, out, y, b, r, st);
+        case OSTR_BF16:
+            return launch_out<KIND, bf16_t>(out_dtype, g, views, a0, a1, cos_weight, img, out, y, b, r, st);
+        case OSTR_F16:
+            return launch_out<KIND, _Float16>(out_dtype, g, views, a0, a1, cos_weight, img, out, y, b, r, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_project(const DevGeom& g, const ViewTab* views, int a0, int a1, int kind,
+                          int cos_weight, const void* img, int img_dtype, void* out,
+                          int out_dtype, const float* y, const float* b, const float* r,
+                          hipStream_t st) {
+    switch (kind) {
+        case 0:
+            return launch_in<0>(img_dtype, out_dtype, g, views, a0, a1, cos_weight, img, out, y, b, r, st);
+        case 1:
+            return launch_kind<1, float, float>(g, views, a0, a1, 0, nullptr, out, y, b, r, st);
+        case 2:
+            return launch_kind<2, float, float>(g, views, a0, a1, 0, nullptr, out, y, b, r, st);
+        case 3:
+            return launch_in<3>(img_dtype, out_dtype, g, views, a0, a1, 1, img, out, y, b, r, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_count_nnz(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                            unsigned long long* counter, hipStream_t st) {
+    const int n_slots = a1 - a0;
+    const int tiles_i = (g.nbx + TILE_I - 1) / TILE_I;
+    const int tiles_j = (g.nby + TILE_J - 1) / TILE_J;
+    const int64_t nblocks = (int64_t)tiles_i * tiles_j * n_slots;
+    if (nblocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(count_kernel, dim3((unsigned)nblocks), dim3(256), 0, st, g, views, a0,
+                       n_slots, tiles_i, tiles_j, counter);
+    return hipGetLastError();
+}
+
+}  // namespace ostr

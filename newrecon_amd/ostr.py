@@ -1,0 +1,390 @@
+"""OSTR DBT reconstruction host: ctypes over libostr_hip.so.
+
+This module is the Python host of the path (the role ostr.py / the notebook
+driver cell and Tomo_OSTR_gold.cpp play in the reference). PyTorch is used
+only for device buffers and the current HIP stream; all arithmetic runs in the
+gfx950 kernels of libostr_hip.so. If the library is missing the import of
+`lib()` raises: there is no CPU fallback.
+
+Two levels:
+* `DBTSystem` - one library handle (geometry + view table) with the
+  reference's launcher and host-op names (fprojectCB_1R_GPU_OSTR_cos,
+  bprojectCB_GPU_SBP, prior_GPU_OSTR, compute_h, update_est, ...), same
+  argument meaning: device tensors in, results written or accumulated in
+  place, half-open view ranges [angleStart, angleEnd).
+* `OSTR` - the reconstruction driver of Tomo_OSTR_gold.cpp:301-918: the
+  precompute of gamma/yry/d, the OS regroup, and per subset forward
+  projection -> h -> backprojection -> prior -> update. `fused=False` issues
+  the reference's launch sequence one operation at a time; `fused=True` uses
+  the fused forward+h and backprojection+prior+update kernels (same
+  arithmetic, two launches per subset).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import time
+from typing import Callable, List, Optional
+
+import numpy as np
+import torch
+
+from .geometry import GAMMA_MODES, POTENTIALS, Geometry, OSTRConfig, regroup_index
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libostr_hip.so")
+
+F32, BF16, F16 = 0, 1, 2
+_DT = {torch.float32: F32, torch.bfloat16: BF16, torch.float16: F16}
+
+
+class OSTRError(RuntimeError):
+    """A libostr_hip call returned an error code."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libostr_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                           "`python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float
+    from .geometry import OstrGeom
+    sig = {
+        "ostr_create": [ctypes.POINTER(OstrGeom), vp, vp, i32, ctypes.POINTER(vp)],
+        "ostr_set_index": [vp, vp],
+        "ostr_fproj": [vp, vp, i32, vp, i32, i32, i32, i32, vp],
+        "ostr_gamma": [vp, vp, i32, i32, i32, vp],
+        "ostr_bproj_sbp": [vp, vp, vp, i32, i32, i32, vp],
+        "ostr_prior": [vp, vp, vp, vp, i32, i32, f32, f32, vp],
+        "ostr_yry": [vp, vp, vp, vp, i64, vp],
+        "ostr_mul": [vp, vp, vp, vp, i64, vp],
+        "ostr_compute_h": [vp, vp, vp, vp, vp, vp, i64, vp],
+        "ostr_update_est": [vp, vp, vp, vp, vp, vp, f32, f32, i32, vp],
+        "ostr_regroup": [vp, vp, vp, i32, vp],
+        "ostr_precompute_d": [vp, vp, vp, vp, i32, vp],
+        "ostr_fproj_h": [vp, vp, i32, vp, vp, vp, vp, i32, i32, i32, vp],
+        "ostr_update_fused": [vp, vp, vp, i32, vp, vp, i32, i32, i32, f32, f32, i32, f32, f32,
+                              i32, vp],
+        "ostr_count_nnz": [vp, i32, i32, ctypes.POINTER(i64), vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.ostr_destroy.argtypes = [vp]
+    L.ostr_destroy.restype = None
+    L.ostr_last_error.argtypes = [vp]
+    L.ostr_last_error.restype = ctypes.c_char_p
+    L.ostr_n_views.argtypes = [vp]
+    L.ostr_n_views.restype = i32
+    _lib = L
+    return L
+
+
+def _p(t: Optional[torch.Tensor]):
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensors passed to libostr_hip must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _dt(t: torch.Tensor) -> int:
+    try:
+        return _DT[t.dtype]
+    except KeyError:
+        raise TypeError(f"unsupported dtype {t.dtype}") from None
+
+
+def _f32(t: torch.Tensor, what: str):
+    if t.dtype != torch.float32:
+        raise TypeError(f"{what} must be float32")
+
+
+class DBTSystem:
+    """One geometry + view table on one device (a libostr_hip handle)."""
+
+    def __init__(self, geom: Geometry, angles, index=None, device=None):
+        geom.validate()
+        self.geom = geom
+        self.angles = np.ascontiguousarray(np.asarray(angles, np.float32))
+        self.n_views = int(self.angles.shape[0])
+        self.device = torch.device(device if device is not None else "cuda")
+        self._L = lib()
+        idx = None if index is None else np.ascontiguousarray(np.asarray(index, np.int32))
+        self.index = np.arange(self.n_views, dtype=np.int32) if idx is None else idx.copy()
+        h = ctypes.c_void_p()
+        self._gc = geom.to_c()
+        with torch.cuda.device(self.device):
+            rc = self._L.ostr_create(ctypes.byref(self._gc), self.angles.ctypes.data_as(ctypes.c_void_p),
+                                     None if idx is None else idx.ctypes.data_as(ctypes.c_void_p),
+                                     self.n_views, ctypes.byref(h))
+        if rc != 0:
+            raise OSTRError(rc, self._L.ostr_last_error(None).decode())
+        self._h = h
+
+    # -- plumbing ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ostr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise OSTRError(rc, self._L.ostr_last_error(self._h).decode())
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def volume(self, dtype=torch.float32, fill=0.0):
+        return torch.full(self.geom.vol_shape, fill, dtype=dtype, device=self.device)
+
+    def sinogram(self, n_slots: int, dtype=torch.float32):
+        return torch.zeros((n_slots,) + self.geom.view_shape, dtype=dtype, device=self.device)
+
+    def set_index(self, index):
+        idx = np.ascontiguousarray(np.asarray(index, np.int32))
+        self._check(self._L.ostr_set_index(self._h, idx.ctypes.data_as(ctypes.c_void_p)))
+        self.index = idx.copy()
+
+    # -- projector family ---------------------------------------------------
+    def fproj(self, img, a0, a1, prj=None, cos_weight=True, prj_dtype=torch.float32):
+        prj = self.sinogram(a1 - a0, prj_dtype) if prj is None else prj
+        self._check(self._L.ostr_fproj(self._h, _p(img), _dt(img), _p(prj), _dt(prj), a0, a1,
+                                       int(cos_weight), self._stream()))
+        return prj
+
+    def gamma(self, a0, a1, mode="ray_length", out=None):
+        out = self.sinogram(a1 - a0) if out is None else out
+        _f32(out, "gamma")
+        self._check(self._L.ostr_gamma(self._h, _p(out), a0, a1, GAMMA_MODES[mode],
+                                       self._stream()))
+        return out
+
+    def bproj_sbp(self, prj, a0, a1, img=None):
+        img = self.volume() if img is None else img
+        _f32(img, "backprojection output")
+        self._check(self._L.ostr_bproj_sbp(self._h, _p(img), _p(prj), _dt(prj), a0, a1,
+                                           self._stream()))
+        return img
+
+    def prior(self, img, potential="hyperbola", delta=5e-4, z_xy_ratio=None, rdd=None,
+              rd=None):
+        rho = self.geom.dz / self.geom.dx if z_xy_ratio is None else z_xy_ratio
+        rdd = self.volume() if rdd is None else rdd
+        rd = self.volume() if rd is None else rd
+        self._check(self._L.ostr_prior(self._h, _p(rdd), _p(rd), _p(img), _dt(img),
+                                       POTENTIALS[potential], delta, rho, self._stream()))
+        return rdd, rd
+
+    def count_nnz(self, a0, a1) -> int:
+        out = ctypes.c_int64()
+        self._check(self._L.ostr_count_nnz(self._h, a0, a1, ctypes.byref(out), self._stream()))
+        return int(out.value)
+
+    # -- fused --------------------------------------------------------------
+    def precompute_d(self, y, r=None, gamma_mode="row_sum", d=None):
+        d = self.volume() if d is None else d
+        self._check(self._L.ostr_precompute_d(self._h, _p(d), _p(y), _p(r),
+                                              GAMMA_MODES[gamma_mode], self._stream()))
+        return d
+
+    def fproj_h(self, img, y, b, r, a0, a1, hsub=None, prj_dtype=torch.float32):
+        hsub = self.sinogram(a1 - a0, prj_dtype) if hsub is None else hsub
+        self._check(self._L.ostr_fproj_h(self._h, _p(img), _dt(img), _p(y), _p(b), _p(r),
+                                         _p(hsub), _dt(hsub), a0, a1, self._stream()))
+        return hsub
+
+    def update_fused(self, x_in, x_out, d, hsub, a0, a1, beta, capl_scale, potential,
+                     delta, z_xy_ratio=None, guard=True):
+        rho = self.geom.dz / self.geom.dx if z_xy_ratio is None else z_xy_ratio
+        self._check(self._L.ostr_update_fused(
+            self._h, _p(x_in), _p(x_out), _dt(x_in), _p(d), _p(hsub), _dt(hsub), a0, a1, beta,
+            capl_scale, POTENTIALS[potential], delta, rho, int(guard), self._stream()))
+        return x_out
+
+    # -- reference names (Tomo_OSTR.cu launch wrappers, gold.cpp host ops) ----
+    def fprojectCB_1R_GPU_OSTR_cos(self, estbuf, prj_est, angleStart, angleEnd):
+        """Tomo_OSTR.cu:2334: prj_est[a-angleStart] = cos t * A_a estbuf."""
+        return self.fproj(estbuf, angleStart, angleEnd, prj=prj_est, cos_weight=True)
+
+    def fprojectCB_1R_GPU_OSTR_normprj(self, d_normprj, angleStart, angleEnd,
+                                       mode="ray_length"):
+        """Tomo_OSTR.cu:2361 (mode ray_length) / ostr.ipynb normprj (row_sum)."""
+        return self.gamma(angleStart, angleEnd, mode, out=d_normprj)
+
+    def bprojectCB_GPU_SBP(self, d_objbuf, d_prjbuf, angleStart, angleEnd):
+        """Tomo_OSTR.cu:2435: d_objbuf += SBP(d_prjbuf) over [angleStart, angleEnd)."""
+        return self.bproj_sbp(d_prjbuf, angleStart, angleEnd, img=d_objbuf)
+
+    def prior_GPU_OSTR(self, d_RDD, d_RD, d_est, delta, z_xy_ratio, potential="lange"):
+        """Tomo_OSTR.cu:2216 (Lange); potential='hyperbola' for the hyperbola prior."""
+        if delta == 0:
+            raise OSTRError(-1, "delta cannot be zero")
+        return self.prior(d_est, potential, delta, z_xy_ratio, rdd=d_RDD, rd=d_RD)
+
+    def prior_GPU_OSTR_Q2(self, d_RDD, d_RD, d_est, z_xy_ratio):
+        """Tomo_OSTR.cu:2189."""
+        return self.prior(d_est, "q2", 1.0, z_xy_ratio, rdd=d_RDD, rd=d_RD)
+
+    def compute_yry(self, yry, prj, scat=None):
+        """Tomo_OSTR_gold.cpp:183: yry = (y-r)^2/y (y-r<=0 -> y; y==0 -> 0)."""
+        self._check(self._L.ostr_yry(self._h, _p(yry), _p(prj), _p(scat), prj.numel(),
+                                     self._stream()))
+        return yry
+
+    def compute_gamma_yry(self, gamma_yry, yry, gamma):
+        """Tomo_OSTR_gold.cpp:208: gamma_yry = yry * gamma."""
+        self._check(self._L.ostr_mul(self._h, _p(gamma_yry), _p(yry), _p(gamma), yry.numel(),
+                                     self._stream()))
+        return gamma_yry
+
+    def compute_h(self, h_sub, prj_sub, blank_sub, line_sub, scat_sub=None):
+        """Tomo_OSTR_gold.cpp:245: h = (y/(b e^-l + r) - 1) b e^-l."""
+        self._check(self._L.ostr_compute_h(self._h, _p(h_sub), _p(prj_sub), _p(blank_sub),
+                                           _p(line_sub), _p(scat_sub), prj_sub.numel(),
+                                           self._stream()))
+        return h_sub
+
+    def update_est(self, est, capL, RD, d, RDD, beta, capl_scale=1.0, guard=False):
+        """Tomo_OSTR_gold.cpp:265 (+ the capL*M scaling of :769)."""
+        self._check(self._L.ostr_update_est(self._h, _p(est), _p(capL), _p(RD), _p(d), _p(RDD),
+                                            beta, capl_scale, int(guard), self._stream()))
+        return est
+
+    def regroup_prj(self, src, n_subsets, dst=None):
+        """Tomo_OSTR_gold.cpp:220: subset-major reorder of [V][nbx][nby]."""
+        dst = torch.empty_like(src) if dst is None else dst
+        self._check(self._L.ostr_regroup(self._h, _p(dst), _p(src), n_subsets, self._stream()))
+        return dst
+
+
+class OSTR:
+    """OSTR driver (Tomo_OSTR_gold.cpp:301-918) on device tensors.
+
+    y, b, r: measured counts, blank scan, scatter as [V][nbx][nby] float32 in
+    natural view order (r may be None = 0). x0: initial image (None = zeros,
+    the reference's effective start, Tomo_OSTR_gold.cpp:578).
+    """
+
+    def __init__(self, geom: Geometry, angles, y, b, r=None, cfg: Optional[OSTRConfig] = None,
+                 x0=None, img_dtype=torch.float32, prj_dtype=torch.float32, device=None):
+        self.cfg = cfg or OSTRConfig()
+        self.geom = geom
+        nv = len(angles)
+        M = self.cfg.n_subsets
+        if M <= 0 or nv % M:
+            raise ValueError(f"ANGLES % subset_number != 0 ({nv} % {M})")
+        if geom.dx != geom.dy:
+            raise ValueError("voxel size in x must equal voxel size in y (prior weights)")
+        if self.cfg.prior != "q2" and self.cfg.delta == 0:
+            raise ValueError("delta cannot be zero")
+        self.sys = DBTSystem(geom, angles, device=device)
+        self.dev = self.sys.device
+        self.M, self.vs = M, nv // M
+        self.img_dtype, self.prj_dtype = img_dtype, prj_dtype
+        self.rho = geom.dz / geom.dx
+
+        def dev32(a):
+            if a is None:
+                return None
+            t = torch.as_tensor(a)
+            return t.to(device=self.dev, dtype=torch.float32).contiguous()
+
+        self.y_nat, self.b_nat, self.r_nat = dev32(y), dev32(b), dev32(r)
+        self.x = (self.sys.volume(img_dtype) if x0 is None
+                  else dev32(x0).to(img_dtype).contiguous())
+        self.iteration = 0
+        self._precomputed = False
+
+    def precompute(self):
+        """gamma, yry, d = SBP(gamma*yry) (natural order), then the OS regroup."""
+        s, cfg = self.sys, self.cfg
+        nv = self.sys.n_views
+        if cfg.fused:
+            self.d = s.precompute_d(self.y_nat, self.r_nat, cfg.gamma_mode)
+        else:
+            gam = s.gamma(0, nv, cfg.gamma_mode)
+            yry = s.compute_yry(torch.empty_like(self.y_nat), self.y_nat, self.r_nat)
+            s.compute_gamma_yry(gam, yry, gam)
+            self.d = s.bprojectCB_GPU_SBP(s.volume(), gam, 0, nv)
+            del gam, yry
+        self.y = s.regroup_prj(self.y_nat, self.M)
+        self.b = s.regroup_prj(self.b_nat, self.M)
+        self.r = None if self.r_nat is None else s.regroup_prj(self.r_nat, self.M)
+        s.set_index(regroup_index(nv, self.M))
+        if cfg.fused:
+            self.h = s.sinogram(self.vs, self.prj_dtype)
+            self.x_next = torch.empty_like(self.x)
+        else:
+            self.line = s.sinogram(self.vs)
+            self.h = s.sinogram(self.vs)
+            self.capL = s.volume()
+            self.RD, self.RDD = s.volume(), s.volume()
+        self._precomputed = True
+
+    def _subset(self, m: int):
+        s, cfg = self.sys, self.cfg
+        a0, a1 = m * self.vs, (m + 1) * self.vs
+        sl = slice(a0, a1)
+        r_sub = None if self.r is None else self.r[sl]
+        if cfg.fused:
+            s.fproj_h(self.x, self.y[sl], self.b[sl], r_sub, a0, a1, hsub=self.h)
+            s.update_fused(self.x, self.x_next, self.d, self.h, a0, a1, cfg.beta, float(self.M),
+                           cfg.prior, cfg.delta, self.rho, cfg.guard)
+            self.x, self.x_next = self.x_next, self.x
+        else:
+            s.fprojectCB_1R_GPU_OSTR_cos(self.x, self.line, a0, a1)
+            s.compute_h(self.h, self.y[sl], self.b[sl], self.line, r_sub)
+            self.capL.zero_()
+            s.bprojectCB_GPU_SBP(self.capL, self.h, a0, a1)
+            self.RDD.zero_()
+            self.RD.zero_()
+            s.prior(self.x, cfg.prior, cfg.delta if cfg.prior != "q2" else 1.0, self.rho,
+                    rdd=self.RDD, rd=self.RD)
+            s.update_est(self.x, self.capL, self.RD, self.d, self.RDD, cfg.beta, float(self.M),
+                         cfg.guard)
+
+    def iterate(self, n_iters: int = 1, callback: Optional[Callable] = None):
+        """Run n_iters full iterations (all subsets); callback(it, x) after each."""
+        if not self._precomputed:
+            self.precompute()
+        for _ in range(n_iters):
+            for m in range(self.M):
+                self._subset(m)
+            self.iteration += 1
+            if callback is not None:
+                callback(self.iteration, self.x)
+        return self.x
+
+    def run(self):
+        return self.iterate(self.cfg.n_iters)
+
+
+def save_raw(path: str, vol: torch.Tensor):
+    """Raw float32 [z][y][x] (the reference's output format, Tomo_OSTR_gold.cpp:65-91)."""
+    vol.detach().to("cpu", torch.float32).contiguous().numpy().tofile(path)
+
+
+def load_raw(path: str, geom: Geometry) -> np.ndarray:
+    """Inverse of save_raw; also the `ini_guess` reader (Tomo_OSTR_gold.cpp:581)."""
+    a = np.fromfile(path, dtype=np.float32)
+    if a.size != geom.n_vox:
+        raise ValueError(f"{path}: {a.size} floats, expected {geom.n_vox}")
+    return a.reshape(geom.vol_shape)

@@ -1,0 +1,226 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg import
+this module; the product path (newrecon_amd) never does. All arrays are numpy,
+C-contiguous; volumes are (nz, ny, nx) and sinograms (slots, nbx, nby).
+The functions mirror the reference (file:line in oracle/ostr_oracle.cpp).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from newrecon_amd.geometry import Geometry, OstrGeom
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libostr_oracle.so")
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_gp = ctypes.POINTER(OstrGeom)
+_i, _i64, _f, _d, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from oracle import build as _b
+            _b.build()
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "orc_fproj_f32": (_i, [_gp, _f32p, _i32p, _f32p, _f32p, _i, _i, _i, _i]),
+            "orc_fproj_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i, _i]),
+            "orc_gamma_f32": (_i, [_gp, _f32p, _i32p, _f32p, _i, _i, _i, _i]),
+            "orc_gamma_f64": (_i, [_gp, _f32p, _i32p, _f64p, _i, _i, _i, _i]),
+            "orc_count_nnz": (_i64, [_gp, _f32p, _i32p, _i, _i, _i]),
+            "orc_walk_f64": (_d, [_gp, _vp, _f64p, _f64p, ctypes.POINTER(_i), ctypes.POINTER(_d)]),
+            "orc_walk_f32": (_f, [_gp, _vp, _f32p, _f32p, ctypes.POINTER(_i), ctypes.POINTER(_f)]),
+            "orc_bproj_sbp_f32": (_i, [_gp, _f32p, _i32p, _f32p, _f32p, _i, _i, _i]),
+            "orc_bproj_sbp_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i]),
+            "orc_bproj_matched_f32": (_i, [_gp, _f32p, _i32p, _f32p, _f32p, _i, _i, _i]),
+            "orc_bproj_matched_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i]),
+            "orc_prior_f32": (_i, [_gp, _f32p, _f32p, _f32p, _i, _f, _f, _i]),
+            "orc_prior_f64": (_i, [_gp, _f64p, _f64p, _f64p, _i, _d, _d, _i]),
+            "orc_penalty_f64": (_d, [_gp, _f64p, _i, _d, _d]),
+            "orc_yry_f32": (None, [_f32p, _f32p, _vp, _i64]),
+            "orc_yry_f64": (None, [_f64p, _f64p, _vp, _i64]),
+            "orc_compute_h_f32": (None, [_f32p, _f32p, _f32p, _f32p, _vp, _i64]),
+            "orc_compute_h_f64": (None, [_f64p, _f64p, _f64p, _f64p, _vp, _i64]),
+            "orc_update_est_f32": (None, [_f32p, _f32p, _f32p, _f32p, _f32p, _f, _f, _i, _i64]),
+            "orc_update_est_f64": (None, [_f64p, _f64p, _f64p, _f64p, _f64p, _d, _d, _i, _i64]),
+            "orc_regroup_f32": (None, [_f32p, _f32p, _i, _i, _i64]),
+            "orc_objective_f64": (_d, [_gp, _f32p, _i, _f64p, _f64p, _f64p, _vp, _i, _d, _d, _i]),
+            "orc_run_f32": (_i, [_gp, _f32p, _i, _f32p, _f32p, _vp, _vp, _i, _i, _d, _d, _i, _i,
+                                 _i, _i, _f32p, _vp]),
+            "orc_run_f64": (_i, [_gp, _f32p, _i, _f64p, _f64p, _vp, _vp, _i, _i, _d, _d, _i, _i,
+                                 _i, _i, _f64p, _vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dt(dtype):
+    return "f64" if np.dtype(dtype) == np.float64 else "f32"
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _idx(index, n):
+    return _c(np.arange(n) if index is None else index, np.int32)
+
+
+def fproj(g: Geometry, angles, img, a0, a1, index=None, cos_weight=True, dtype=np.float32,
+          nthreads=0):
+    img = _c(img, dtype)
+    out = np.empty((a1 - a0, g.nbx, g.nby), dtype)
+    getattr(lib(), "orc_fproj_" + _dt(dtype))(
+        ctypes.byref(g.to_c()), _c(angles, np.float32), _idx(index, len(angles)), img.ravel(),
+        out.ravel(), a0, a1, int(cos_weight), nthreads)
+    return out
+
+
+def gamma(g: Geometry, angles, a0, a1, mode, index=None, dtype=np.float32, nthreads=0):
+    out = np.empty((a1 - a0, g.nbx, g.nby), dtype)
+    getattr(lib(), "orc_gamma_" + _dt(dtype))(
+        ctypes.byref(g.to_c()), _c(angles, np.float32), _idx(index, len(angles)), out.ravel(),
+        a0, a1, int(mode), nthreads)
+    return out
+
+
+def count_nnz(g: Geometry, angles, a0, a1, index=None, nthreads=0) -> int:
+    return int(lib().orc_count_nnz(ctypes.byref(g.to_c()), _c(angles, np.float32),
+                                   _idx(index, len(angles)), a0, a1, nthreads))
+
+
+def walk(g: Geometry, src, dst, img=None, dtype=np.float64):
+    """One Siddon ray; returns (sum, hit, length). img=None -> row sum."""
+    hit = ctypes.c_int()
+    if np.dtype(dtype) == np.float64:
+        length = ctypes.c_double()
+        im = None if img is None else _c(img, np.float64)
+        s = lib().orc_walk_f64(ctypes.byref(g.to_c()), _ptr(im), _c(src, np.float64),
+                               _c(dst, np.float64), ctypes.byref(hit), ctypes.byref(length))
+    else:
+        length = ctypes.c_float()
+        im = None if img is None else _c(img, np.float32)
+        s = lib().orc_walk_f32(ctypes.byref(g.to_c()), _ptr(im), _c(src, np.float32),
+                               _c(dst, np.float32), ctypes.byref(hit), ctypes.byref(length))
+    return s, bool(hit.value), length.value
+
+
+def bproj_sbp(g: Geometry, angles, prj, a0, a1, index=None, img=None, dtype=np.float32,
+              nthreads=0):
+    """img += SBP(prj) over slots [a0,a1); returns img."""
+    img = np.zeros(g.vol_shape, dtype) if img is None else _c(img, dtype).copy()
+    getattr(lib(), "orc_bproj_sbp_" + _dt(dtype))(
+        ctypes.byref(g.to_c()), _c(angles, np.float32), _idx(index, len(angles)), img.ravel(),
+        _c(prj, dtype).ravel(), a0, a1, nthreads)
+    return img
+
+
+def bproj_matched(g: Geometry, angles, prj, a0, a1, index=None, cos_weight=True, img=None,
+                  dtype=np.float64):
+    """img += (cos A)^T prj, the exact transpose of fproj."""
+    img = np.zeros(g.vol_shape, dtype) if img is None else _c(img, dtype).copy()
+    getattr(lib(), "orc_bproj_matched_" + _dt(dtype))(
+        ctypes.byref(g.to_c()), _c(angles, np.float32), _idx(index, len(angles)), img.ravel(),
+        _c(prj, dtype).ravel(), a0, a1, int(cos_weight))
+    return img
+
+
+def prior(g: Geometry, x, potential, delta, rho=None, rdd=None, rd=None, dtype=np.float32,
+          nthreads=0):
+    """(rdd, rd) += prior curvature/gradient of x."""
+    rho = g.dz / g.dx if rho is None else rho
+    rdd = np.zeros(g.vol_shape, dtype) if rdd is None else _c(rdd, dtype).copy()
+    rd = np.zeros(g.vol_shape, dtype) if rd is None else _c(rd, dtype).copy()
+    rc = getattr(lib(), "orc_prior_" + _dt(dtype))(
+        ctypes.byref(g.to_c()), rdd.ravel(), rd.ravel(), _c(x, dtype).ravel(), int(potential),
+        delta, rho, nthreads)
+    if rc != 0:
+        raise ValueError("oracle prior: invalid argument")
+    return rdd, rd
+
+
+def penalty(g: Geometry, x, potential, delta, rho=None) -> float:
+    rho = g.dz / g.dx if rho is None else rho
+    return lib().orc_penalty_f64(ctypes.byref(g.to_c()), _c(x, np.float64).ravel(),
+                                 int(potential), delta, rho)
+
+
+def yry(y, r=None, dtype=np.float32):
+    y = _c(y, dtype)
+    out = np.empty_like(y)
+    rr = None if r is None else _c(r, dtype)
+    getattr(lib(), "orc_yry_" + _dt(dtype))(out.ravel(), y.ravel(), _ptr(rr), y.size)
+    return out
+
+
+def compute_h(y, b, l, r=None, dtype=np.float32):
+    y, b, l = _c(y, dtype), _c(b, dtype), _c(l, dtype)
+    out = np.empty_like(y)
+    rr = None if r is None else _c(r, dtype)
+    getattr(lib(), "orc_compute_h_" + _dt(dtype))(out.ravel(), y.ravel(), b.ravel(), l.ravel(),
+                                                  _ptr(rr), y.size)
+    return out
+
+
+def update_est(x, capl, rd, d, rdd, beta, scale, guard=True, dtype=np.float32):
+    x = _c(x, dtype).copy()
+    getattr(lib(), "orc_update_est_" + _dt(dtype))(
+        x.ravel(), _c(capl, dtype).ravel(), _c(rd, dtype).ravel(), _c(d, dtype).ravel(),
+        _c(rdd, dtype).ravel(), beta, scale, int(guard), x.size)
+    return x
+
+
+def regroup(src, n_subsets):
+    src = _c(src, np.float32)
+    dst = np.empty_like(src)
+    nv = src.shape[0]
+    lib().orc_regroup_f32(dst.ravel(), src.ravel(), nv, n_subsets, src[0].size)
+    return dst
+
+
+def objective(g: Geometry, angles, x, y, b, r=None, potential=2, beta=0.0, delta=1.0,
+              nthreads=0) -> float:
+    """FP64 penalized negative Poisson log-likelihood (all views, natural order)."""
+    rr = None if r is None else _c(r, np.float64)
+    return lib().orc_objective_f64(
+        ctypes.byref(g.to_c()), _c(angles, np.float32), len(angles), _c(x, np.float64).ravel(),
+        _c(y, np.float64).ravel(), _c(b, np.float64).ravel(), _ptr(rr), int(potential), beta,
+        delta, nthreads)
+
+
+def run(g: Geometry, angles, y, b, r=None, x0=None, n_subsets=3, n_iters=10, beta=1e3,
+        delta=5e-4, potential=2, gamma_mode=1, guard=True, dtype=np.float32, snapshots=False,
+        nthreads=0):
+    """Full OSTR driver (Tomo_OSTR_gold.cpp:301-918 order). Returns (x, snaps|None)."""
+    nv = len(angles)
+    y, b = _c(y, dtype), _c(b, dtype)
+    rr = None if r is None else _c(r, dtype)
+    xi = None if x0 is None else _c(x0, dtype)
+    out = np.empty(g.vol_shape, dtype)
+    snaps = np.empty((n_iters,) + g.vol_shape, dtype) if snapshots else None
+    rc = getattr(lib(), "orc_run_" + _dt(dtype))(
+        ctypes.byref(g.to_c()), _c(angles, np.float32), nv, y.ravel(), b.ravel(), _ptr(rr),
+        _ptr(xi), n_subsets, n_iters, beta, delta, int(potential), int(gamma_mode), int(guard),
+        nthreads, out.ravel(), _ptr(snaps))
+    if rc != 0:
+        raise ValueError(f"oracle run: invalid configuration (code {rc})")
+    return out, snaps

@@ -1,0 +1,53 @@
+"""The C ABI library loads on the CPU host and exports every symbol declared
+in include/ostr_hip.h (no compute calls: there may be no GPU)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ostr_hip.h")
+LIB = os.path.join(ROOT, "newrecon_amd", "libostr_hip.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ostr_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_surface():
+    names = declared_functions()
+    for required in ("ostr_create", "ostr_destroy", "ostr_last_error", "ostr_fproj",
+                     "ostr_gamma", "ostr_bproj_sbp", "ostr_prior", "ostr_update_est",
+                     "ostr_fproj_h", "ostr_update_fused", "ostr_precompute_d"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(LIB):
+        pytest.skip("libostr_hip.so not built")
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    # and nothing else leaks from the C++ side
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True, text=True)
+    exported = {l.split()[-1] for l in out.stdout.splitlines() if " T " in l}
+    extra = sorted(e for e in exported if e.startswith("ostr_") and e not in declared_functions())
+    assert not extra, extra
+
+
+def test_error_path_without_device():
+    """Argument validation happens before any device call."""
+    if not os.path.exists(LIB):
+        pytest.skip("libostr_hip.so not built")
+    lib = ctypes.CDLL(LIB)
+    lib.ostr_create.restype = ctypes.c_int
+    lib.ostr_last_error.restype = ctypes.c_char_p
+    lib.ostr_last_error.argtypes = [ctypes.c_void_p]
+    h = ctypes.c_void_p()
+    rc = lib.ostr_create(None, None, None, 0, ctypes.byref(h))
+    assert rc == -1 and h.value is None
+    assert b"geometry" in lib.ostr_last_error(None)

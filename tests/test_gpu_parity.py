@@ -1,0 +1,193 @@
+"""HIP kernels (through the C ABI) vs the CPU oracle on identical inputs.
+
+Bars: per-kernel max error relative to the oracle's peak <= 1e-5 (FP32; the
+walker and elementwise ops share the oracle's expression graph, the SBP and
+prior use hardware reciprocal / rsqrt within 2 ulp); OSTR per-iteration
+relative RMSE <= 1e-4 (the parity bar of the path); exact integer equality for
+the nnz count and the regroup permutation.
+"""
+import numpy as np
+import pytest
+import torch
+
+from newrecon_amd.geometry import OSTRConfig, regroup_index, scan_small, scan_tiny
+from newrecon_amd.ostr import OSTR, DBTSystem, OSTRError
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_max(a, ref):
+    return float(np.abs(np.asarray(a, np.float64) - ref).max() / max(np.abs(ref).max(), 1e-30))
+
+
+def rel_rmse(a, ref):
+    ref = np.asarray(ref, np.float64)
+    return float(np.linalg.norm(np.asarray(a, np.float64) - ref) / max(np.linalg.norm(ref), 1e-30))
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev)
+
+
+@pytest.fixture(scope="module")
+def sysm(small_case, dev):
+    s = small_case[0]
+    return DBTSystem(s.geom, s.angles, device=dev)
+
+
+def test_fproj_matches_oracle(small_case, sysm, dev, oracle_lib):
+    s, x, y, b = small_case
+    for cos_w in (True, False):
+        ref = oracle_lib.fproj(s.geom, s.angles, x, 0, s.n_views, cos_weight=cos_w)
+        got = sysm.fproj(T(x, dev), 0, s.n_views, cos_weight=cos_w).cpu().numpy()
+        assert rel_max(got, ref) <= 1e-6, rel_max(got, ref)
+        # rays that miss the volume are exactly zero on both sides
+        assert np.array_equal(got == 0, ref == 0)
+
+
+def test_fproj_subset_slots_follow_index(small_case, dev, oracle_lib):
+    s, x, _, _ = small_case
+    idx = regroup_index(s.n_views, 3)
+    sysr = DBTSystem(s.geom, s.angles, index=idx, device=dev)
+    for m in range(3):
+        a0, a1 = 3 * m, 3 * m + 3
+        ref = oracle_lib.fproj(s.geom, s.angles, x, a0, a1, index=idx)
+        got = sysr.fproj(T(x, dev), a0, a1).cpu().numpy()
+        assert rel_max(got, ref) <= 1e-6
+
+
+@pytest.mark.parametrize("mode", ["ray_length", "row_sum"])
+def test_gamma_matches_oracle(small_case, sysm, mode, oracle_lib):
+    s = small_case[0]
+    ref = oracle_lib.gamma(s.geom, s.angles, 0, s.n_views, {"ray_length": 0, "row_sum": 1}[mode])
+    got = sysm.gamma(0, s.n_views, mode).cpu().numpy()
+    assert rel_max(got, ref) <= 1e-6
+
+
+def test_nnz_count_exact(small_case, sysm, oracle_lib):
+    s = small_case[0]
+    for a0, a1 in ((0, 3), (0, s.n_views)):
+        assert sysm.count_nnz(a0, a1) == oracle_lib.count_nnz(s.geom, s.angles, a0, a1)
+
+
+def test_sbp_matches_oracle(small_case, sysm, dev, oracle_lib):
+    s, x, y, b = small_case
+    rng = np.random.default_rng(5)
+    prj = rng.standard_normal((s.n_views,) + s.geom.view_shape).astype(np.float32)
+    base = rng.standard_normal(s.geom.vol_shape).astype(np.float32)
+    ref = oracle_lib.bproj_sbp(s.geom, s.angles, prj, 0, s.n_views, img=base)
+    got = sysm.bproj_sbp(T(prj, dev), 0, s.n_views, img=T(base, dev)).cpu().numpy()
+    assert rel_max(got, ref) <= 1e-5, rel_max(got, ref)
+    # sub-range of slots with a regrouped order accumulates the same way
+    idx = regroup_index(s.n_views, 3)
+    sysr = DBTSystem(s.geom, s.angles, index=idx, device=dev)
+    ref = oracle_lib.bproj_sbp(s.geom, s.angles, prj[:3], 3, 6, index=idx)
+    got = sysr.bproj_sbp(T(prj[:3], dev), 3, 6).cpu().numpy()
+    assert rel_max(got, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("potential,code", [("q2", 0), ("lange", 1), ("hyperbola", 2)])
+def test_prior_matches_oracle(small_case, sysm, dev, potential, code, oracle_lib):
+    s, x, _, _ = small_case
+    rng = np.random.default_rng(6)
+    xn = (x + 1e-3 * rng.standard_normal(x.shape)).astype(np.float32)
+    delta = 5e-4
+    rdd0 = rng.random(x.shape).astype(np.float32)
+    rd0 = rng.standard_normal(x.shape).astype(np.float32)
+    r_dd, r_d = oracle_lib.prior(s.geom, xn, code, delta, rdd=rdd0, rd=rd0)
+    g_dd, g_d = sysm.prior(T(xn, dev), potential, delta, rdd=T(rdd0, dev), rd=T(rd0, dev))
+    assert rel_max(g_dd.cpu().numpy(), r_dd) <= 1e-5
+    assert rel_max(g_d.cpu().numpy(), r_d) <= 1e-5
+
+
+def test_elementwise_ops(small_case, sysm, dev, oracle_lib):
+    s, x, y, b = small_case
+    rng = np.random.default_rng(7)
+    r = (rng.random(y.shape) * 50).astype(np.float32)
+    yy = y.copy()
+    yy.ravel()[::97] = 0.0
+    got = sysm.compute_yry(torch.empty_like(T(yy, dev)), T(yy, dev), T(r, dev)).cpu().numpy()
+    assert np.array_equal(got, oracle_lib.yry(yy, r))
+    l = rng.random(y.shape).astype(np.float32) * 3
+    got = sysm.compute_h(torch.empty_like(T(y, dev)), T(y, dev), T(b, dev), T(l, dev),
+                         T(r, dev)).cpu().numpy()
+    assert rel_max(got, oracle_lib.compute_h(y, b, l, r)) <= 1e-6
+    nv = x.size
+    capl, rd, d, rdd = (rng.standard_normal(x.shape).astype(np.float32) for _ in range(4))
+    d = np.abs(d) + 0.1
+    rdd = np.abs(rdd)
+    est = np.abs(rng.standard_normal(x.shape)).astype(np.float32)
+    got = sysm.update_est(T(est, dev), T(capl, dev), T(rd, dev), T(d, dev), T(rdd, dev), 1e3,
+                          3.0).cpu().numpy()
+    ref = oracle_lib.update_est(est, capl, rd, d, rdd, 1e3, 3.0)
+    assert np.array_equal(got, ref)
+    reg = sysm.regroup_prj(T(y, dev), 3).cpu().numpy()
+    assert np.array_equal(reg, oracle_lib.regroup(y, 3))
+
+
+def test_precompute_d_matches_oracle(small_case, sysm, dev, oracle_lib):
+    s, x, y, b = small_case
+    for mode, code in (("row_sum", 1), ("ray_length", 0)):
+        gam = oracle_lib.gamma(s.geom, s.angles, 0, s.n_views, code)
+        gc = oracle_lib.yry(y) * gam
+        ref = oracle_lib.bproj_sbp(s.geom, s.angles, gc, 0, s.n_views)
+        got = sysm.precompute_d(T(y, dev), None, mode).cpu().numpy()
+        assert rel_max(got, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_ostr_small_per_iteration_parity(small_case, dev, fused, oracle_lib):
+    """Plain FP32 OSTR on the 200x100x20 / 9-view workload: <= 1e-4 rel RMSE
+    against the FP32 oracle after every one of 10 iterations."""
+    s, x, y, b = small_case
+    cfg = OSTRConfig(n_subsets=3, n_iters=10, beta=1e3, delta=5e-4, prior="hyperbola",
+                     gamma_mode="row_sum", guard=True, fused=fused)
+    _, snaps = oracle_lib.run(s.geom, s.angles, y, b, n_subsets=3, n_iters=10, beta=1e3,
+                              delta=5e-4, potential=2, gamma_mode=1, guard=True, snapshots=True)
+    rec = OSTR(s.geom, s.angles, y, b, cfg=cfg, device=dev)
+    errs = []
+    rec.iterate(10, callback=lambda it, xg: errs.append(rel_rmse(xg.cpu().numpy(), snaps[it - 1])))
+    assert max(errs) <= 1e-4, errs
+
+
+def test_fused_equals_unfused(small_case, dev):
+    s, x, y, b = small_case
+    out = []
+    for fused in (False, True):
+        cfg = OSTRConfig(n_subsets=3, n_iters=3, fused=fused)
+        out.append(OSTR(s.geom, s.angles, y, b, cfg=cfg, device=dev).iterate(3).cpu().numpy())
+    assert rel_max(out[1], out[0]) <= 1e-6
+
+
+@pytest.mark.parametrize("prior,gmode", [("lange", "ray_length"), ("q2", "row_sum")])
+def test_ostr_reference_modes(small_case, dev, prior, gmode, oracle_lib):
+    """The reference's own settings: Lange prior (Recon_Method 1) with gamma =
+    ray length (Tomo_OSTR.cu:1172), and Q2 (Recon_Method 0)."""
+    s, x, y, b = small_case
+    code = {"q2": 0, "lange": 1}[prior]
+    cfg = OSTRConfig(n_subsets=3, n_iters=4, beta=1e3, delta=5e-4, prior=prior,
+                     gamma_mode=gmode, guard=True, fused=True)
+    ref, _ = oracle_lib.run(s.geom, s.angles, y, b, n_subsets=3, n_iters=4, beta=1e3, delta=5e-4,
+                            potential=code, gamma_mode={"ray_length": 0, "row_sum": 1}[gmode])
+    got = OSTR(s.geom, s.angles, y, b, cfg=cfg, device=dev).iterate(4).cpu().numpy()
+    assert rel_rmse(got, ref) <= 1e-4
+
+
+def test_tiny_and_edge_cases(dev, oracle_lib):
+    # ragged detector/volume sizes that do not fill a work-group tile
+    s = scan_tiny(nx=13, ny=7, nz=5, nbx=37, nby=9, n_views=3)
+    from newrecon_amd.phantom import breast_phantom
+    x = breast_phantom(s.geom, 3, (2, 3), 1)
+    sysm = DBTSystem(s.geom, s.angles, device=dev)
+    ref = oracle_lib.fproj(s.geom, s.angles, x, 0, 3)
+    got = sysm.fproj(T(x, dev), 0, 3).cpu().numpy()
+    assert rel_max(got, ref) <= 1e-6
+    # empty view range is a no-op
+    out = torch.full((1,) + s.geom.view_shape, 7.0, device=dev)
+    sysm.fproj(T(x, dev), 1, 1, prj=out)
+    assert float(out.min()) == 7.0
+    # invalid ranges are reported, not executed
+    with pytest.raises(OSTRError):
+        sysm.fproj(T(x, dev), 2, 5)
+    with pytest.raises(OSTRError):
+        sysm.prior(T(x, dev), "lange", 0.0)
