@@ -312,6 +312,25 @@ class OSTR:
                   else dev32(x0).to(img_dtype).contiguous())
         self.iteration = 0
         self._precomputed = False
+        # optional per-kernel timing: name -> list of (start, end) torch.cuda.Events
+        self.events = None
+
+    def _timed(self, name, fn, *args, **kw):
+        if self.events is None:
+            return fn(*args, **kw)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out = fn(*args, **kw)
+        e1.record()
+        self.events.setdefault(name, []).append((e0, e1))
+        return out
+
+    def kernel_times_ms(self):
+        """Average duration (ms) per timed launch name; needs a synchronize first."""
+        if not self.events:
+            return {}
+        return {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in self.events.items()}
 
     def precompute(self):
         """gamma, yry, d = SBP(gamma*yry) (natural order), then the OS regroup."""
@@ -345,9 +364,10 @@ class OSTR:
         sl = slice(a0, a1)
         r_sub = None if self.r is None else self.r[sl]
         if cfg.fused:
-            s.fproj_h(self.x, self.y[sl], self.b[sl], r_sub, a0, a1, hsub=self.h)
-            s.update_fused(self.x, self.x_next, self.d, self.h, a0, a1, cfg.beta, float(self.M),
-                           cfg.prior, cfg.delta, self.rho, cfg.guard)
+            self._timed("fproj_h", s.fproj_h, self.x, self.y[sl], self.b[sl], r_sub, a0, a1,
+                        hsub=self.h)
+            self._timed("update_fused", s.update_fused, self.x, self.x_next, self.d, self.h, a0,
+                        a1, cfg.beta, float(self.M), cfg.prior, cfg.delta, self.rho, cfg.guard)
             self.x, self.x_next = self.x_next, self.x
         else:
             s.fprojectCB_1R_GPU_OSTR_cos(self.x, self.line, a0, a1)
