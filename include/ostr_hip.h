@@ -135,12 +135,25 @@ int ostr_compute_h(ostr_handle* h, float* hsub, const float* y, const float* b,
 int ostr_update_est(ostr_handle* h, float* est, const float* capL, const float* rd,
                     const float* d, const float* rdd, float beta, float capl_scale,
                     int32_t guard, void* stream);
-/* Subset-major regroup of a [n_views][nbx*nby] f32 array: dst[m][j] =
- * src[j*M + m] (regroup_prj, Tomo_OSTR_gold.cpp:220-242). dst != src. */
+/* Subset-major regroup of a [n_views][nbx][nby] f32 array: view block m*Vs+j
+ * of dst = view block j*M+m of src (regroup_prj, Tomo_OSTR_gold.cpp:220-242).
+ * to_rows != 0 also transposes every view to the detector-row-major layout
+ * [nby][nbx] used by the fused path. dst != src. */
 int ostr_regroup(ostr_handle* h, float* dst, const float* src, int32_t n_subsets,
+                 int32_t to_rows, void* stream);
+/* [n_slots][nbx][nby] -> detector-row-major [n_slots][nby][nbx] (no
+ * reference counterpart: layout conversion for the fused path). */
+int ostr_to_rows(ostr_handle* h, void* dst, const void* src, int32_t n_slots, int32_t dtype,
                  void* stream);
 
 /* ---------------------------------------------------------- fused path -- */
+/* The fused path keeps subset sinograms (y, b, r, h) in the detector-row-major
+ * layout [slot][nby][nbx] (x fastest), so that the 64 lanes of a wave, placed
+ * on consecutive voxels along x, read consecutive bins. */
+
+/* Same as ostr_bproj_sbp with prj already detector-row-major. */
+int ostr_bproj_sbp_rows(ostr_handle* h, float* img_accum, const void* prj, int32_t prj_dtype,
+                        int32_t a0, int32_t a1, void* stream);
 
 /* d = SBP_all(gamma * yry) over all n_views slots, written (not accumulated)
  * to d (f32). Fuses ostr_gamma + ostr_yry + ostr_mul + ostr_bproj_sbp of the
@@ -151,14 +164,16 @@ int ostr_precompute_d(ostr_handle* h, float* d, const float* y, const float* r,
                       int32_t gamma_mode, void* stream);
 
 /* Forward projection of slots [a0,a1) with the h epilogue: hsub = h(y, b, r,
- * cos_t * A x). y, b, r are the subset's slices [(a-a0)][nbx][nby] (r may be
- * NULL); hsub in prj_dtype; y/b/r in f32. Fuses ostr_fproj + ostr_compute_h. */
+ * cos_t * A x). y, b, r are the subset's slices, detector-row-major
+ * [(a-a0)][nby][nbx] (r may be NULL); hsub (same layout) in prj_dtype; y/b/r
+ * in f32. Fuses ostr_fproj + ostr_compute_h. */
 int ostr_fproj_h(ostr_handle* h, const void* img, int32_t img_dtype, const float* y,
                  const float* b, const float* r, void* hsub, int32_t prj_dtype,
                  int32_t a0, int32_t a1, void* stream);
 
 /* x_out = max(0, x_in - (M*SBP(hsub) + beta*RD(x_in)) / (d + 2 beta RDD(x_in)))
- * in one pass: SBP gather of the subset's h, 27-neighbour prior on x_in
+ * in one pass (hsub detector-row-major, as written by ostr_fproj_h): SBP
+ * gather of the subset's h, 27-neighbour prior on x_in
  * (Jacobi: the prior sees the pre-update image, Tomo_OSTR_gold.cpp:775 before
  * :814) and the update. x_in and x_out must not alias. */
 int ostr_update_fused(ostr_handle* h, const void* x_in, void* x_out, int32_t img_dtype,

@@ -26,8 +26,8 @@ struct ostr_handle {
     ostr::ViewTab* views_dev = nullptr;
     ostr::ViewTab* views_nat_dev = nullptr;
     unsigned long long* counter = nullptr;
-    float* work = nullptr;
-    size_t work_elems = 0;
+    void* work = nullptr;     // grow-on-demand device work buffer
+    size_t work_bytes = 0;
     std::string err;
 };
 
@@ -92,6 +92,23 @@ int upload_views(ostr_handle* h) {
     hipError_t e = hipMemcpy(h->views_dev, h->views_host.data(),
                              sizeof(ostr::ViewTab) * h->n_views, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(h, e, "upload view table");
+    return OSTR_OK;
+}
+
+// Grow the handle's device work buffer to at least `bytes` (stream-ordered:
+// the old buffer may still be read by queued work on `st`).
+int ensure_work(ostr_handle* h, size_t bytes, hipStream_t st, const char* fn) {
+    if (h->work_bytes >= bytes) return OSTR_OK;
+    if (h->work) {
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(h, e, fn);
+        (void)hipFree(h->work);
+        h->work = nullptr;
+        h->work_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&h->work, bytes);
+    if (e != hipSuccess) return hip_fail(h, e, fn);
+    h->work_bytes = bytes;
     return OSTR_OK;
 }
 
@@ -262,9 +279,31 @@ OSTR_API int ostr_bproj_sbp(ostr_handle* h, float* img_accum, const void* prj,
         return rc;
     if (!img_accum || !prj) return fail(h, OSTR_E_ARG, "ostr_bproj_sbp: NULL buffer");
     if ((rc = use_device(h))) return rc;
-    hipError_t e = ostr::launch_sbp(h->dg, h->views_host.data(), a0, a1, img_accum, prj,
-                                    prj_dtype, (hipStream_t)stream);
+    if (a0 == a1) return OSTR_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = (int64_t)(a1 - a0) * h->geom.nbx * h->geom.nby;
+    if ((rc = ensure_work(h, n * ostr::elem_size(prj_dtype), st, "ostr_bproj_sbp"))) return rc;
+    // the kernel reads detector-row-major bins: transpose the reference layout first
+    hipError_t e = ostr::launch_transpose_bins(prj, h->work, a1 - a0, h->geom.nbx,
+                                               h->geom.nby, prj_dtype, st);
+    if (e == hipSuccess)
+        e = ostr::launch_sbp_rows(h->dg, h->views_host.data(), a0, a1, img_accum, h->work,
+                                  prj_dtype, st);
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_bproj_sbp");
+}
+
+OSTR_API int ostr_bproj_sbp_rows(ostr_handle* h, float* img_accum, const void* prj,
+                                 int32_t prj_dtype, int32_t a0, int32_t a1, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_bproj_sbp_rows")) ||
+        (rc = check_dtype(h, prj_dtype, "ostr_bproj_sbp_rows")))
+        return rc;
+    if (!img_accum || !prj) return fail(h, OSTR_E_ARG, "ostr_bproj_sbp_rows: NULL buffer");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_sbp_rows(h->dg, h->views_host.data(), a0, a1, img_accum, prj,
+                                         prj_dtype, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_bproj_sbp_rows");
 }
 
 OSTR_API int ostr_prior(ostr_handle* h, float* rdd, float* rd, const void* img,
@@ -332,7 +371,7 @@ OSTR_API int ostr_update_est(ostr_handle* h, float* est, const float* capL, cons
 }
 
 OSTR_API int ostr_regroup(ostr_handle* h, float* dst, const float* src, int32_t n_subsets,
-                          void* stream) {
+                          int32_t to_rows, void* stream) {
     CHECK_HANDLE(h);
     if (!dst || !src || dst == src) return fail(h, OSTR_E_ARG, "ostr_regroup: bad buffers");
     if (n_subsets <= 0 || h->n_views % n_subsets)
@@ -340,9 +379,34 @@ OSTR_API int ostr_regroup(ostr_handle* h, float* dst, const float* src, int32_t 
                     h->n_views, n_subsets);
     int rc;
     if ((rc = use_device(h))) return rc;
-    hipError_t e = ostr::launch_regroup(dst, src, h->n_views, n_subsets,
-                                        (int64_t)h->geom.nbx * h->geom.nby, (hipStream_t)stream);
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nbin = (int64_t)h->geom.nbx * h->geom.nby;
+    hipError_t e = hipSuccess;
+    if (!to_rows) {
+        e = ostr::launch_regroup(dst, src, h->n_views, n_subsets, nbin, st);
+    } else {
+        const int vs = h->n_views / n_subsets;
+        for (int k = 0; k < h->n_views && e == hipSuccess; ++k) {
+            const int m = k / vs, j = k % vs;
+            e = ostr::launch_transpose_bins(src + (int64_t)(j * n_subsets + m) * nbin,
+                                            dst + (int64_t)k * nbin, 1, h->geom.nbx,
+                                            h->geom.nby, OSTR_F32, st);
+        }
+    }
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_regroup");
+}
+
+OSTR_API int ostr_to_rows(ostr_handle* h, void* dst, const void* src, int32_t n_slots,
+                          int32_t dtype, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_dtype(h, dtype, "ostr_to_rows"))) return rc;
+    if (!dst || !src || dst == src || n_slots < 0)
+        return fail(h, OSTR_E_ARG, "ostr_to_rows: bad buffers");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_transpose_bins(src, dst, n_slots, h->geom.nbx, h->geom.nby,
+                                               dtype, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_to_rows");
 }
 
 // ---------------------------------------------------------------- fused --
@@ -356,29 +420,17 @@ OSTR_API int ostr_precompute_d(ostr_handle* h, float* d, const float* y, const f
     if ((rc = use_device(h))) return rc;
     hipStream_t st = (hipStream_t)stream;
     const int64_t nall = (int64_t)h->n_views * h->geom.nbx * h->geom.nby;
-    if (h->work_elems < (size_t)nall) {
-        if (h->work) {
-            (void)hipStreamSynchronize(st);
-            (void)hipFree(h->work);
-            h->work = nullptr;
-            h->work_elems = 0;
-        }
-        hipError_t e = hipMalloc(&h->work, sizeof(float) * nall);
-        if (e != hipSuccess) return hip_fail(h, e, "ostr_precompute_d: work buffer");
-        h->work_elems = nall;
-    }
+    if ((rc = ensure_work(h, nall * sizeof(float), st, "ostr_precompute_d"))) return rc;
     const int64_t nvox = (int64_t)h->geom.nx * h->geom.ny * h->geom.nz;
     // gamma and d use the natural view order (computed before the OS regroup,
-    // Tomo_OSTR_gold.cpp:450-538)
+    // Tomo_OSTR_gold.cpp:450-538); gamma*yry goes to the work buffer row-major
     hipError_t e = ostr::launch_project(h->dg, h->views_nat_dev, 0, h->n_views,
-                                        gamma_mode == OSTR_GAMMA_ROW_SUM ? 2 : 1, 0, nullptr,
-                                        OSTR_F32, h->work, OSTR_F32, nullptr, nullptr, nullptr,
-                                        st);
-    if (e == hipSuccess) e = ostr::launch_yry_gamma(h->work, y, r, nall, st);
+                                        gamma_mode == OSTR_GAMMA_ROW_SUM ? 5 : 4, 0, nullptr,
+                                        OSTR_F32, h->work, OSTR_F32, y, nullptr, r, st);
     if (e == hipSuccess) e = ostr::launch_fill(d, 0.0f, nvox, st);
     if (e == hipSuccess)
-        e = ostr::launch_sbp(h->dg, h->views_nat.data(), 0, h->n_views, d, h->work, OSTR_F32,
-                             st);
+        e = ostr::launch_sbp_rows(h->dg, h->views_nat.data(), 0, h->n_views, d, h->work,
+                                  OSTR_F32, st);
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_precompute_d");
 }
 

@@ -70,7 +70,8 @@ __device__ __forceinline__ void st<_Float16>(_Float16* p, int64_t i, float v) {
 
 // ---- launchers (return hipError_t of the launch) -------------------------
 // Forward family. kind: 0 = projection, 1 = gamma ray length, 2 = gamma row
-// sum, 3 = projection + h epilogue.
+// sum (reference layout out), 3 = projection + h epilogue (row-major y/b/r/h),
+// 4 / 5 = gamma ray length / row sum times yry (row-major out).
 hipError_t launch_project(const DevGeom& g, const ViewTab* views, int a0, int a1, int kind,
                           int cos_weight, const void* img, int img_dtype, void* out,
                           int out_dtype, const float* y, const float* b, const float* r,
@@ -78,9 +79,14 @@ hipError_t launch_project(const DevGeom& g, const ViewTab* views, int a0, int a1
 hipError_t launch_count_nnz(const DevGeom& g, const ViewTab* views, int a0, int a1,
                             unsigned long long* counter, hipStream_t st);
 
-// Backprojection family.
-hipError_t launch_sbp(const DevGeom& g, const ViewTab* views, int a0, int a1,
-                      float* img_accum, const void* prj, int prj_dtype, hipStream_t st);
+// Backprojection family. prj / hsub are detector-row-major [slot][nby][nbx];
+// `views` is the host copy of the slot table.
+hipError_t launch_sbp_rows(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                           float* img_accum, const void* prj, int prj_dtype, hipStream_t st);
+// [slot][nbx][nby] -> [slot][nby][nbx] (dtype selects 4- or 2-byte elements).
+hipError_t launch_transpose_bins(const void* src, void* dst, int n_slots, int nbx, int nby,
+                                 int dtype, hipStream_t st);
+size_t elem_size(int dtype);
 hipError_t launch_update_fused(const DevGeom& g, const ViewTab* views, int a0, int a1,
                                const void* x_in, void* x_out, int img_dtype, const float* d,
                                const void* hsub, int prj_dtype, float beta, float capl_scale,

@@ -177,7 +177,11 @@ __device__ __forceinline__ float walk(const DevGeom& g, const TI* __restrict__ i
     return sum;
 }
 
-// kind: 0 projection, 1 gamma ray length, 2 gamma row sum, 3 projection + h.
+// kind: 0 projection, 1 gamma ray length, 2 gamma row sum (reference layout
+// [slot][nbx][nby] out); 3 projection + h epilogue (y, b, r and h in
+// detector-row-major layout [slot][nby][nbx]); 4 / 5 gamma (ray length / row
+// sum) times yry(y, r) with y, r read in the reference layout and the product
+// written row-major (the fused precompute).
 template <int KIND, typename TI, typename TO>
 __global__ __launch_bounds__(256) void project_kernel(DevGeom g, const ViewTab* __restrict__ views,
                                                       int a0, int n_slots, int tiles_i,
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(256) void project_kernel(DevGeom g, const ViewTab* 
     float s;
     if (KIND == 0 || KIND == 3)
         s = walk<false, false, TI>(g, img, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
-    else if (KIND == 2)
+    else if (KIND == 2 || KIND == 5)
         s = walk<true, false, TI>(g, img, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
     else {
         walk<true, false, TI>(g, img, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
@@ -221,14 +225,26 @@ __global__ __launch_bounds__(256) void project_kernel(DevGeom g, const ViewTab* 
         val = s * v.c;
     else
         val = s;
-    const int64_t o = ((int64_t)slot * g.nbx + i) * g.nby + j;
+    const int64_t o_ref = ((int64_t)slot * g.nbx + i) * g.nby + j;
+    const int64_t o_row = ((int64_t)slot * g.nby + j) * g.nbx + i;
     if (KIND == 3) {
         // h epilogue (compute_h, Tomo_OSTR_gold.cpp:255-258)
-        float yt = b[o] * expf(-val);
-        float den = r ? yt + r[o] : yt;
-        val = (y[o] / den - 1.0f) * yt;
+        float yt = b[o_row] * expf(-val);
+        float den = r ? yt + r[o_row] : yt;
+        val = (y[o_row] / den - 1.0f) * yt;
     }
-    st<TO>(out, o, val);
+    if (KIND == 4 || KIND == 5) {
+        // yry * gamma (compute_yry :183-205, compute_gamma_yry :208-218)
+        const float yv = y[o_ref];
+        float c = 0.0f;
+        if (yv != 0.0f) {
+            float dif = yv - (r ? r[o_ref] : 0.0f);
+            if (dif <= 0.0f) dif = yv;
+            c = (dif * dif) / yv;
+        }
+        val = c * val;
+    }
+    st<TO>(out, (KIND >= 3) ? o_row : o_ref, val);
 }
 
 __global__ __launch_bounds__(256) void count_kernel(DevGeom g, const ViewTab* __restrict__ views,
@@ -318,6 +334,10 @@ hipError_t launch_project(const DevGeom& g, const ViewTab* views, int a0, int a1
             return launch_kind<2, float, float>(g, views, a0, a1, 0, nullptr, out, y, b, r, st);
         case 3:
             return launch_in<3>(img_dtype, out_dtype, g, views, a0, a1, 1, img, out, y, b, r, st);
+        case 4:
+            return launch_kind<4, float, float>(g, views, a0, a1, 0, nullptr, out, y, b, r, st);
+        case 5:
+            return launch_kind<5, float, float>(g, views, a0, a1, 0, nullptr, out, y, b, r, st);
     }
     return hipErrorInvalidValue;
 }

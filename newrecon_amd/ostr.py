@@ -70,7 +70,9 @@ def lib():
         "ostr_mul": [vp, vp, vp, vp, i64, vp],
         "ostr_compute_h": [vp, vp, vp, vp, vp, vp, i64, vp],
         "ostr_update_est": [vp, vp, vp, vp, vp, vp, f32, f32, i32, vp],
-        "ostr_regroup": [vp, vp, vp, i32, vp],
+        "ostr_regroup": [vp, vp, vp, i32, i32, vp],
+        "ostr_to_rows": [vp, vp, vp, i32, i32, vp],
+        "ostr_bproj_sbp_rows": [vp, vp, vp, i32, i32, i32, vp],
         "ostr_precompute_d": [vp, vp, vp, vp, i32, vp],
         "ostr_fproj_h": [vp, vp, i32, vp, vp, vp, vp, i32, i32, i32, vp],
         "ostr_update_fused": [vp, vp, vp, i32, vp, vp, i32, i32, i32, f32, f32, i32, f32, f32,
@@ -268,11 +270,31 @@ class DBTSystem:
                                             beta, capl_scale, int(guard), self._stream()))
         return est
 
-    def regroup_prj(self, src, n_subsets, dst=None):
-        """Tomo_OSTR_gold.cpp:220: subset-major reorder of [V][nbx][nby]."""
-        dst = torch.empty_like(src) if dst is None else dst
-        self._check(self._L.ostr_regroup(self._h, _p(dst), _p(src), n_subsets, self._stream()))
+    def regroup_prj(self, src, n_subsets, dst=None, to_rows=False):
+        """Tomo_OSTR_gold.cpp:220: subset-major reorder of [V][nbx][nby]
+        (to_rows: output detector-row-major [V][nby][nbx])."""
+        if dst is None:
+            shape = (src.shape[0], self.geom.nby, self.geom.nbx) if to_rows else src.shape
+            dst = torch.empty(shape, dtype=src.dtype, device=src.device)
+        self._check(self._L.ostr_regroup(self._h, _p(dst), _p(src), n_subsets, int(to_rows),
+                                         self._stream()))
         return dst
+
+    def to_rows(self, src, dst=None):
+        """[S][nbx][nby] -> detector-row-major [S][nby][nbx]."""
+        dst = (torch.empty((src.shape[0], self.geom.nby, self.geom.nbx), dtype=src.dtype,
+                           device=src.device) if dst is None else dst)
+        self._check(self._L.ostr_to_rows(self._h, _p(dst), _p(src), src.shape[0], _dt(src),
+                                         self._stream()))
+        return dst
+
+    def bproj_sbp_rows(self, prj_rows, a0, a1, img=None):
+        """SBP of a detector-row-major sinogram [S][nby][nbx], accumulated into img."""
+        img = self.volume() if img is None else img
+        _f32(img, "backprojection output")
+        self._check(self._L.ostr_bproj_sbp_rows(self._h, _p(img), _p(prj_rows), _dt(prj_rows),
+                                                a0, a1, self._stream()))
+        return img
 
 
 class OSTR:
@@ -344,12 +366,15 @@ class OSTR:
             s.compute_gamma_yry(gam, yry, gam)
             self.d = s.bprojectCB_GPU_SBP(s.volume(), gam, 0, nv)
             del gam, yry
-        self.y = s.regroup_prj(self.y_nat, self.M)
-        self.b = s.regroup_prj(self.b_nat, self.M)
-        self.r = None if self.r_nat is None else s.regroup_prj(self.r_nat, self.M)
+        # the fused kernels read subset sinograms detector-row-major
+        rows = cfg.fused
+        self.y = s.regroup_prj(self.y_nat, self.M, to_rows=rows)
+        self.b = s.regroup_prj(self.b_nat, self.M, to_rows=rows)
+        self.r = None if self.r_nat is None else s.regroup_prj(self.r_nat, self.M, to_rows=rows)
         s.set_index(regroup_index(nv, self.M))
         if cfg.fused:
-            self.h = s.sinogram(self.vs, self.prj_dtype)
+            self.h = torch.zeros((self.vs, self.geom.nby, self.geom.nbx), dtype=self.prj_dtype,
+                                 device=self.dev)
             self.x_next = torch.empty_like(self.x)
         else:
             self.line = s.sinogram(self.vs)

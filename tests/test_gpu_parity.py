@@ -86,6 +86,21 @@ def test_sbp_matches_oracle(small_case, sysm, dev, oracle_lib):
     assert rel_max(got, ref) <= 1e-5
 
 
+def test_row_major_layout_entry_points(small_case, sysm, dev, oracle_lib):
+    """to_rows is an exact transpose; SBP of the row-major copy equals SBP of
+    the reference layout; regroup(to_rows) = transpose of regroup."""
+    s, x, y, b = small_case
+    rng = np.random.default_rng(8)
+    prj = rng.standard_normal((s.n_views,) + s.geom.view_shape).astype(np.float32)
+    rows = sysm.to_rows(T(prj, dev)).cpu().numpy()
+    assert np.array_equal(rows, prj.transpose(0, 2, 1))
+    a = sysm.bproj_sbp(T(prj, dev), 0, s.n_views).cpu().numpy()
+    r = sysm.bproj_sbp_rows(T(rows, dev), 0, s.n_views).cpu().numpy()
+    assert np.array_equal(a, r)
+    reg = sysm.regroup_prj(T(y, dev), 3, to_rows=True).cpu().numpy()
+    assert np.array_equal(reg, oracle_lib.regroup(y, 3).transpose(0, 2, 1))
+
+
 @pytest.mark.parametrize("potential,code", [("q2", 0), ("lange", 1), ("hyperbola", 2)])
 def test_prior_matches_oracle(small_case, sysm, dev, potential, code, oracle_lib):
     s, x, _, _ = small_case
