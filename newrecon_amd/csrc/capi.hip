@@ -122,6 +122,9 @@ int validate_geom(const ostr_geom* g) {
         return fail(nullptr, OSTR_E_ARG, "ostr_create: voxel and bin sizes must be positive");
     if ((int64_t)g->nbx * g->nby > (int64_t)1 << 30)
         return fail(nullptr, OSTR_E_ARG, "ostr_create: detector too large");
+    // 32-bit voxel offsets and buffer-resource sizes: < 2^30 voxels
+    if ((int64_t)g->nx * g->ny * g->nz >= (int64_t)1 << 30)
+        return fail(nullptr, OSTR_E_ARG, "ostr_create: volume must have < 2^30 voxels");
     return OSTR_OK;
 }
 

@@ -68,6 +68,34 @@ __device__ __forceinline__ void st<_Float16>(_Float16* p, int64_t i, float v) {
     p[i] = (_Float16)v;
 }
 
+// Volume gathers through a raw buffer resource: 32-bit byte offsets (no 64-bit
+// address arithmetic per gather) and hardware range checking against the
+// volume size. Word 3 0x00020000 is the raw-buffer descriptor of the gfx9
+// family (gfx90a/gfx94x/gfx950).
+template <typename T>
+struct VolReader {
+    __amdgpu_buffer_rsrc_t rsrc;
+    __device__ VolReader(const T* p, uint32_t n) {
+        rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * sizeof(T)),
+                                                 0x00020000);
+    }
+    __device__ __forceinline__ float operator()(uint32_t i) const;
+};
+template <>
+__device__ __forceinline__ float VolReader<float>::operator()(uint32_t i) const {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, i * 4u, 0, 0));
+}
+template <>
+__device__ __forceinline__ float VolReader<bf16_t>::operator()(uint32_t i) const {
+    return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc, i * 2u, 0, 0)
+                           << 16);
+}
+template <>
+__device__ __forceinline__ float VolReader<_Float16>::operator()(uint32_t i) const {
+    unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(rsrc, i * 2u, 0, 0);
+    return (float)__builtin_bit_cast(_Float16, b);
+}
+
 // ---- launchers (return hipError_t of the launch) -------------------------
 // Forward family. kind: 0 = projection, 1 = gamma ray length, 2 = gamma row
 // sum (reference layout out), 3 = projection + h epilogue (row-major y/b/r/h),

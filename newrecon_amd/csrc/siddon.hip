@@ -23,6 +23,7 @@ namespace {
 
 constexpr int TILE_I = 64;   // bins along detector x per tile (one per lane)
 constexpr int TILE_J = 4;    // bins along detector y per tile (one per wave)
+constexpr int WALK_BATCH = 4;  // walk steps whose gathers are in flight together
 
 __device__ __forceinline__ float bin_centre(float origin, int i, float size) {
     return (float)((double)origin + ((double)i + 0.5) * (double)size);
@@ -105,7 +106,7 @@ __device__ __forceinline__ float stop_threshold(float lmax) {
 // sum (ONES: the plain row sum); hit tells whether the ray meets the volume.
 // COUNT: also counts in-volume segments of positive length into *nseg.
 template <bool ONES, bool COUNT, typename TI>
-__device__ __forceinline__ float walk(const DevGeom& g, const TI* __restrict__ img, float sx,
+__device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img, float sx,
                                       float sy, float sz, float ex, float ey, float ez,
                                       bool& hit, float& len_out, int& nseg) {
     hit = false;
@@ -127,25 +128,20 @@ __device__ __forceinline__ float walk(const DevGeom& g, const TI* __restrict__ i
     axis_entry(Y, lmin, sy, ey, g.y_p0, g.dy, g.ny, g.hp_y, g.last_y);
     axis_entry(Z, lmin, sz, ez, g.z_p0, g.dz, g.nz, g.hp_z, g.last_z);
 
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    int64_t vid = ((int64_t)Z.p * ny + Y.p) * nx + X.p;
-    const int64_t sxs = X.step, sys = (int64_t)Y.step * nx, szs = (int64_t)Z.step * nx * ny;
+    const unsigned nx = g.nx, ny = g.ny, nz = g.nz;
+    // 32-bit voxel offset (the volume is limited to < 2^30 voxels)
+    uint32_t vid = ((uint32_t)Z.p * ny + (uint32_t)Y.p) * nx + (uint32_t)X.p;
+    const uint32_t sxs = (uint32_t)X.step, sys = (uint32_t)(Y.step * (int)nx),
+                   szs = (uint32_t)(Z.step * (int)(nx * ny));
     float sum = 0.0f;
     float cur = lmin;
-    float at = X.next;
-    if (at > Y.next) at = Y.next;
-    if (at > Z.next) at = Z.next;
-
-    const bool fast = lmax > 0.0f;
-    const float thr = fast ? stop_threshold(lmax) : 0.0f;
-    while (fast ? (at < thr) : keep_walking(lmax, at)) {
-        float seg = (at - cur) * len;
-        if ((unsigned)X.p < (unsigned)nx && (unsigned)Y.p < (unsigned)ny &&
-            (unsigned)Z.p < (unsigned)nz) {
-            float v = ONES ? 1.0f : ld<TI>(img, vid);
-            sum = fmaf(v, seg, sum);
-            if (COUNT && seg > 0.0f) ++nseg;
-        }
+    // min of the three next crossings; the reference's cascade of `>` tests
+    // picks the same value (no NaNs reach here)
+    float at = fminf(fminf(X.next, Y.next), Z.next);
+    auto inside = [&]() {
+        return (unsigned)X.p < nx && (unsigned)Y.p < ny && (unsigned)Z.p < nz;
+    };
+    auto advance = [&]() {
         if (at == X.next) {
             X.p += X.step;
             X.next = X.next + X.inc;
@@ -162,14 +158,58 @@ __device__ __forceinline__ float walk(const DevGeom& g, const TI* __restrict__ i
             vid += szs;
         }
         cur = at;
-        at = X.next;
-        if (at > Y.next) at = Y.next;
-        if (at > Z.next) at = Z.next;
+        at = fminf(fminf(X.next, Y.next), Z.next);
+    };
+
+    if (lmax > 0.0f) {
+        // Steps are taken in batches of WALK_BATCH: the geometry of a batch is
+        // advanced first, then its gathers are issued together and the sums
+        // accumulated in step order, so a batch pays one memory latency.
+        const float thr = stop_threshold(lmax);
+        bool active = at < thr;
+        while (active) {
+            float seg[WALK_BATCH];
+            uint32_t off[WALK_BATCH];
+            bool ok[WALK_BATCH];
+#pragma unroll
+            for (int k = 0; k < WALK_BATCH; ++k) {
+                ok[k] = false;
+                seg[k] = 0.0f;
+                off[k] = 0;
+                if (active) {
+                    seg[k] = (at - cur) * len;
+                    ok[k] = inside();
+                    off[k] = vid;
+                    advance();
+                    active = at < thr;
+                }
+            }
+            float v[WALK_BATCH];
+#pragma unroll
+            for (int k = 0; k < WALK_BATCH; ++k) v[k] = (!ONES && ok[k]) ? img(off[k]) : 1.0f;
+#pragma unroll
+            for (int k = 0; k < WALK_BATCH; ++k) {
+                if (ok[k]) {
+                    sum = fmaf(v[k], seg[k], sum);
+                    if (COUNT && seg[k] > 0.0f) ++nseg;
+                }
+            }
+        }
+    } else {
+        // lmax <= 0 (volume behind the source): the literal relative test
+        while (keep_walking(lmax, at)) {
+            float seg = (at - cur) * len;
+            if (inside()) {
+                float v = ONES ? 1.0f : img(vid);
+                sum = fmaf(v, seg, sum);
+                if (COUNT && seg > 0.0f) ++nseg;
+            }
+            advance();
+        }
     }
     float seg = (lmax - cur) * len;
-    if ((unsigned)X.p < (unsigned)nx && (unsigned)Y.p < (unsigned)ny &&
-        (unsigned)Z.p < (unsigned)nz) {
-        float v = ONES ? 1.0f : ld<TI>(img, vid);
+    if (inside()) {
+        float v = ONES ? 1.0f : img(vid);
         sum = fmaf(v, seg, sum);
         if (COUNT && seg > 0.0f) ++nseg;
     }
@@ -210,12 +250,13 @@ __global__ __launch_bounds__(256) void project_kernel(DevGeom g, const ViewTab* 
     float len;
     int nseg = 0;
     float s;
+    const VolReader<TI> vol(img, (uint32_t)g.nx * g.ny * g.nz);
     if (KIND == 0 || KIND == 3)
-        s = walk<false, false, TI>(g, img, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
+        s = walk<false, false, TI>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
     else if (KIND == 2 || KIND == 5)
-        s = walk<true, false, TI>(g, img, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
+        s = walk<true, false, TI>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
     else {
-        walk<true, false, TI>(g, img, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
+        walk<true, false, TI>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
         s = len;
     }
     float val;
@@ -263,7 +304,8 @@ __global__ __launch_bounds__(256) void count_kernel(DevGeom g, const ViewTab* __
         const ViewTab v = views[a0 + slot];
         bool hit;
         float len;
-        walk<true, true, float>(g, (const float*)nullptr, v.x0, g.src_y, v.z0,
+        const VolReader<float> none(nullptr, 0u);
+        walk<true, true, float>(g, none, v.x0, g.src_y, v.z0,
                                 bin_centre(g.x_d0, i, g.dbx), bin_centre(g.y_d0, j, g.dby),
                                 -g.det_r, hit, len, nseg);
     }
