@@ -34,7 +34,7 @@ namespace ostr {
 namespace {
 
 constexpr int MAX_SLOTS = 64;  // views per launch handled in one pass
-constexpr int ROWS = 4;        // voxel rows (y) per wave
+constexpr int ROWS = 8;        // voxel rows (y) per wave
 constexpr int WAVES = 4;       // waves per work-group
 constexpr int TILE_X = 64, TILE_Y = ROWS * WAVES;
 
@@ -71,57 +71,88 @@ __device__ __forceinline__ TileCoord tile_coord(const DevGeom& g) {
     return c;
 }
 
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 // total[r] = sum over slots of the SBP value of voxel (ix, iy0 + r, iz);
-// prj is [slot][nby][nbx].
+// prj is [slot][nby][nbx]. The row terms (jlb, fracJ, y offset, masks) of a
+// (row, slot) pair are wave-uniform: lane r + ROWS*k evaluates them for row r
+// and slot k of a chunk of VIEW_CHUNK slots, and every lane reads them back
+// with readlane, so each exact division runs once per wave.
 template <typename TP>
 __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int n,
                                          const TP* __restrict__ prj, int ix, int iy0, int iz,
                                          float total[ROWS]) {
+    constexpr int VIEW_CHUNK = 64 / ROWS;
     const int64_t nbin = (int64_t)g.nbx * g.nby;
+    const int lane = threadIdx.x & 63;
     const float y0 = g.src_y;
     const float x1 = cell_centre(g.x_p0, ix, g.dx);
     const float z1 = cell_centre(g.z_p0, iz, g.dz);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) total[r] = 0.0f;
-    for (int a = 0; a < n; ++a) {
-        const float x0 = S.x0[a], z0 = S.z0[a], num = S.num[a];
-        if (z1 == z0) continue;
-        const float ex = x0 - x1, ez = z0 - z1;
-        const float exx = ex * ex;
-        const float t = (-g.det_r - z0) / (z1 - z0);
-        const float x2 = fmaf(x1 - x0, t, x0);
-        int il;
-        float fi;
-        tap((x2 - g.x_d0) / g.dbx, il, fi);
-        const bool i0 = (unsigned)il < (unsigned)g.nbx;
-        const bool i1 = (unsigned)(il + 1) < (unsigned)g.nbx;
-        const float omfi = 1.0f - fi;
-        const TP* P = prj + (int64_t)a * nbin;
+    for (int c0 = 0; c0 < n; c0 += VIEW_CHUNK) {
+        // this lane's (row, slot) entry of the chunk
+        int rt_jl = 0;
+        float rt_fj = 0.0f, rt_ey = 0.0f;
+        {
+            const int r = lane % ROWS, a = c0 + lane / ROWS;
+            if (a < n) {
+                const float z0 = S.z0[a];
+                const float y1 = cell_centre(g.y_p0, iy0 + r, g.dy);
+                rt_ey = y0 - y1;
+                if (z1 != z0) {
+                    const float t = (-g.det_r - z0) / (z1 - z0);
+                    const float y2 = fmaf(y1 - y0, t, y0);
+                    tap((y2 - g.y_d0) / g.dby, rt_jl, rt_fj);
+                }
+            }
+        }
+        const int na = n - c0 < VIEW_CHUNK ? n - c0 : VIEW_CHUNK;
+        for (int k = 0; k < na; ++k) {
+            const int a = c0 + k;
+            const float x0 = S.x0[a], z0 = S.z0[a], num = S.num[a];
+            if (z1 == z0) continue;
+            const float ex = x0 - x1, ez = z0 - z1;
+            const float exx = ex * ex;
+            const float t = (-g.det_r - z0) / (z1 - z0);
+            const float x2 = fmaf(x1 - x0, t, x0);
+            int il;
+            float fi;
+            tap((x2 - g.x_d0) / g.dbx, il, fi);
+            const bool i0 = (unsigned)il < (unsigned)g.nbx;
+            const bool i1 = (unsigned)(il + 1) < (unsigned)g.nbx;
+            const float omfi = 1.0f - fi;
+            const TP* P = prj + (int64_t)a * nbin + il;
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-            const float y1 = cell_centre(g.y_p0, iy0 + r, g.dy);
-            const float ey = y0 - y1;
-            const float y2 = fmaf(y1 - y0, t, y0);
-            int jl;
-            float fj;
-            tap((y2 - g.y_d0) / g.dby, jl, fj);
-            const bool j0 = (unsigned)jl < (unsigned)g.nby;
-            const bool j1 = (unsigned)(jl + 1) < (unsigned)g.nby;
-            const float den = fmaf(ez, ez, fmaf(ey, ey, exx));
-            const float dw = num * __builtin_amdgcn_rcpf(den);
-            const TP* row0 = P + (int64_t)jl * g.nbx + il;
-            float p00 = 0.0f, p10 = 0.0f, p01 = 0.0f, p11 = 0.0f;
-            if (i0 && j0) p00 = ld<TP>(row0, 0);
-            if (i1 && j0) p10 = ld<TP>(row0, 1);
-            if (i0 && j1) p01 = ld<TP>(row0, g.nbx);
-            if (i1 && j1) p11 = ld<TP>(row0, g.nbx + 1);
-            float d1 = omfi * p00;
-            d1 = fmaf(fi, p10, d1);
-            float d2 = omfi * p01;
-            d2 = fmaf(fi, p11, d2);
-            float u = fmaf(fj, d2, (1.0f - fj) * d1);
-            u = (u * g.dz) * dw;
-            total[r] = total[r] + u;
+            for (int r = 0; r < ROWS; ++r) {
+                const int src = k * ROWS + r;
+                const int jl = __builtin_amdgcn_readlane(rt_jl, src);
+                const float fj = readlane_f(rt_fj, src);
+                const float ey = readlane_f(rt_ey, src);
+                const bool j0 = (unsigned)jl < (unsigned)g.nby;
+                const bool j1 = (unsigned)(jl + 1) < (unsigned)g.nby;
+                const float den = fmaf(ez, ez, fmaf(ey, ey, exx));
+                const float dw = num * __builtin_amdgcn_rcpf(den);
+                const TP* row0 = P + (int64_t)jl * g.nbx;
+                float p00 = 0.0f, p10 = 0.0f, p01 = 0.0f, p11 = 0.0f;
+                if (j0) {
+                    if (i0) p00 = ld<TP>(row0, 0);
+                    if (i1) p10 = ld<TP>(row0, 1);
+                }
+                if (j1) {
+                    if (i0) p01 = ld<TP>(row0, g.nbx);
+                    if (i1) p11 = ld<TP>(row0, g.nbx + 1);
+                }
+                float d1 = omfi * p00;
+                d1 = fmaf(fi, p10, d1);
+                float d2 = omfi * p01;
+                d2 = fmaf(fi, p11, d2);
+                float u = fmaf(fj, d2, (1.0f - fj) * d1);
+                u = (u * g.dz) * dw;
+                total[r] = total[r] + u;
+            }
         }
     }
 }
@@ -131,9 +162,12 @@ __global__ __launch_bounds__(256) void sbp_kernel(DevGeom g, SlotTab S, int n_sl
                                                   const TP* __restrict__ prj,
                                                   float* __restrict__ img) {
     const TileCoord c = tile_coord(g);
-    if (c.ix >= g.nx || c.iy0 >= g.ny) return;
+    // whole waves past the last row leave; lanes past the last column stay
+    // (they evaluate row terms other lanes read) and only skip the stores
+    if (c.iy0 >= g.ny) return;
     float total[ROWS];
     sbp_rows<TP>(g, S, n_slots, prj, c.ix, c.iy0, c.iz, total);
+    if (c.ix >= g.nx) return;
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         const int iy = c.iy0 + r;
@@ -149,9 +183,10 @@ __global__ __launch_bounds__(256) void update_fused_kernel(
     TI* __restrict__ x_out, const float* __restrict__ d, PriorW W, float inv_delta, float beta,
     float capl_scale, int guard) {
     const TileCoord c = tile_coord(g);
-    if (c.ix >= g.nx || c.iy0 >= g.ny) return;
+    if (c.iy0 >= g.ny) return;  // see sbp_kernel
     float total[ROWS];
     sbp_rows<TP>(g, S, n_slots, hsub, c.ix, c.iy0, c.iz, total);
+    if (c.ix >= g.nx) return;
     const float two_beta = 2.0f * beta;
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
