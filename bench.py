@@ -37,6 +37,22 @@ def algorithmic_bytes(geom, vs, kernel, s_img=4, s_prj=4):
     raise KeyError(kernel)
 
 
+PROFILE_JSON = os.path.join(ROOT, "profiles",
+                            "fproj_h-update_fused_dbt-clinical-1400x600x60-15v.json")
+PROFILED_KERNEL = {"fproj_h": "project_kernel<3, float, float>",
+                   "update_fused": "update_fused_kernel<2, float, float>"}
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (clinical workload), or None."""
+    try:
+        data = json.load(open(PROFILE_JSON))
+        return data["kernels"][PROFILED_KERNEL[kernel]]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(scan, x_host, y_host, b_host, threads):
     """Oracle (C++/OpenMP restatement) timed over subset 0 of one iteration."""
     import numpy as np
@@ -77,6 +93,9 @@ def main(argv=None):
     ap.add_argument("--config", default="clinical", choices=["clinical", "small"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed loop, run 3 plain device copies of 256 MiB (a "
+                         "known-traffic kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -119,6 +138,14 @@ def main(argv=None):
     s_per_iter = t_total / args.steps
     print(f"[bench] {args.steps} iterations: {s_per_iter*1e3:.2f} ms/iter", file=sys.stderr,
           flush=True)
+
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # per-kernel timing pass (events on the launch stream), same iterations
     rec.events = {}
@@ -173,7 +200,9 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": (round(profiled_traffic(dominant))
+                        if args.config == "clinical" and profiled_traffic(dominant) else None),
+            "traffic_source": os.path.relpath(PROFILE_JSON, ROOT),
             "algorithmic_bytes": bytes_dom,
         },
         "cpu_baseline": cpu,
