@@ -76,16 +76,21 @@ __device__ __forceinline__ float readlane_f(float v, int lane) {
 }
 
 // total[r] = sum over slots of the SBP value of voxel (ix, iy0 + r, iz);
-// prj is [slot][nby][nbx]. The row terms (jlb, fracJ, y offset, masks) of a
-// (row, slot) pair are wave-uniform: lane r + ROWS*k evaluates them for row r
-// and slot k of a chunk of VIEW_CHUNK slots, and every lane reads them back
-// with readlane, so each exact division runs once per wave.
+// prj is [slot][nby][nbx], read through a buffer resource with 32-bit element
+// offsets. The row terms (jlb, fracJ, y offset) of a (row, slot) pair are
+// wave-uniform: lane r + ROWS*k evaluates them for row r and slot k of a chunk
+// of VIEW_CHUNK slots, and every lane reads them back with readlane, so each
+// exact division runs once per wave. When every lane's two tap columns and the
+// row's two tap rows lie on the detector (the common case), the four taps are
+// loaded without masks; otherwise each tap is masked on its own, as the
+// reference does.
 template <typename TP>
 __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int n,
-                                         const TP* __restrict__ prj, int ix, int iy0, int iz,
+                                         const VolReader<TP>& P, int ix, int iy0, int iz,
                                          float total[ROWS]) {
     constexpr int VIEW_CHUNK = 64 / ROWS;
-    const int64_t nbin = (int64_t)g.nbx * g.nby;
+    const uint32_t nbin = (uint32_t)g.nbx * (uint32_t)g.nby;
+    const uint32_t nbx = (uint32_t)g.nbx;
     const int lane = threadIdx.x & 63;
     const float y0 = g.src_y;
     const float x1 = cell_centre(g.x_p0, ix, g.dx);
@@ -93,8 +98,7 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) total[r] = 0.0f;
     for (int c0 = 0; c0 < n; c0 += VIEW_CHUNK) {
-        // this lane's (row, slot) entry of the chunk
-        int rt_jl = 0;
+        int rt_jl = -2;
         float rt_fj = 0.0f, rt_ey = 0.0f;
         {
             const int r = lane % ROWS, a = c0 + lane / ROWS;
@@ -123,8 +127,9 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
             tap((x2 - g.x_d0) / g.dbx, il, fi);
             const bool i0 = (unsigned)il < (unsigned)g.nbx;
             const bool i1 = (unsigned)(il + 1) < (unsigned)g.nbx;
+            const bool wave_i_ok = __all(i0 && i1);
             const float omfi = 1.0f - fi;
-            const TP* P = prj + (int64_t)a * nbin + il;
+            const uint32_t base = (uint32_t)a * nbin + (uint32_t)il;
 #pragma unroll
             for (int r = 0; r < ROWS; ++r) {
                 const int src = k * ROWS + r;
@@ -135,15 +140,18 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
                 const bool j1 = (unsigned)(jl + 1) < (unsigned)g.nby;
                 const float den = fmaf(ez, ez, fmaf(ey, ey, exx));
                 const float dw = num * __builtin_amdgcn_rcpf(den);
-                const TP* row0 = P + (int64_t)jl * g.nbx;
-                float p00 = 0.0f, p10 = 0.0f, p01 = 0.0f, p11 = 0.0f;
-                if (j0) {
-                    if (i0) p00 = ld<TP>(row0, 0);
-                    if (i1) p10 = ld<TP>(row0, 1);
-                }
-                if (j1) {
-                    if (i0) p01 = ld<TP>(row0, g.nbx);
-                    if (i1) p11 = ld<TP>(row0, g.nbx + 1);
+                const uint32_t off = base + (uint32_t)jl * nbx;
+                float p00, p10, p01, p11;
+                if (wave_i_ok && j0 && j1) {
+                    p00 = P(off);
+                    p10 = P(off + 1);
+                    p01 = P(off + nbx);
+                    p11 = P(off + nbx + 1);
+                } else {
+                    p00 = (i0 && j0) ? P(off) : 0.0f;
+                    p10 = (i1 && j0) ? P(off + 1) : 0.0f;
+                    p01 = (i0 && j1) ? P(off + nbx) : 0.0f;
+                    p11 = (i1 && j1) ? P(off + nbx + 1) : 0.0f;
                 }
                 float d1 = omfi * p00;
                 d1 = fmaf(fi, p10, d1);
@@ -165,8 +173,9 @@ __global__ __launch_bounds__(256) void sbp_kernel(DevGeom g, SlotTab S, int n_sl
     // whole waves past the last row leave; lanes past the last column stay
     // (they evaluate row terms other lanes read) and only skip the stores
     if (c.iy0 >= g.ny) return;
+    const VolReader<TP> P(prj, (uint32_t)n_slots * g.nbx * g.nby);
     float total[ROWS];
-    sbp_rows<TP>(g, S, n_slots, prj, c.ix, c.iy0, c.iz, total);
+    sbp_rows<TP>(g, S, n_slots, P, c.ix, c.iy0, c.iz, total);
     if (c.ix >= g.nx) return;
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
@@ -177,6 +186,99 @@ __global__ __launch_bounds__(256) void sbp_kernel(DevGeom g, SlotTab S, int n_sl
     }
 }
 
+// Prior terms and update of the ROWS voxels (ix, iy0 + r, iz), r = 0..ROWS-1,
+// from a sliding 3x3x3 window: each step loads the 9 values (3 slices x 3
+// columns) of the next row, so a thread reads 9*(ROWS+2) values instead of
+// 26*ROWS. Terms are summed per voxel in the reference's order (dz outer, dy,
+// dx inner), then the voxel is updated and stored at once. CHECKED: the tile
+// touches the volume boundary; a neighbour outside the volume contributes
+// t = 0 (it is replaced by the voxel itself, Tomo_OSTR.cu:238-241).
+template <int POT, bool CHECKED, typename TI>
+__device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI>& X,
+                                            TI* __restrict__ x_out,
+                                            const float* __restrict__ d, const TileCoord& c,
+                                            const float total[ROWS], const PriorW& W,
+                                            float inv_delta, float beta, float capl_scale,
+                                            int guard) {
+    const int ix = c.ix, iy0 = c.iy0, iz = c.iz;
+    const uint32_t nx = g.nx, plane = (uint32_t)g.nx * g.ny;
+    const float two_beta = 2.0f * beta;
+    bool cx[3], cz[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        cx[q] = (unsigned)(ix + q - 1) < (unsigned)g.nx;
+        cz[q] = (unsigned)(iz + q - 1) < (unsigned)g.nz;
+    }
+    float win[3][3][3];  // [row dy][slice dz][column dx]
+    bool ok[3][3][3];
+    auto load_row = [&](int slot, int y) {
+        const bool cy = (unsigned)y < (unsigned)g.ny;
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz) {
+            const uint32_t o = (uint32_t)(iz + dz - 1) * plane + (uint32_t)y * nx + (uint32_t)ix;
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                if (CHECKED) {
+                    ok[slot][dz][dx] = cy && cz[dz] && cx[dx];
+                    win[slot][dz][dx] = ok[slot][dz][dx] ? X(o + dx - 1) : 0.0f;
+                } else {
+                    win[slot][dz][dx] = X(o + dx - 1);
+                }
+            }
+        }
+    };
+    load_row(0, iy0 - 1);
+    load_row(1, iy0);
+    const bool lane_ok = ix < g.nx;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        load_row(2, iy0 + r + 1);
+        const float xc = win[1][1][1];
+        float rdd = 0.0f, rd = 0.0f;
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) {
+                    const int nidx = dz * 9 + dy * 3 + dx;
+                    if (nidx == 13) continue;
+                    float t = xc - win[dy][dz][dx];
+                    if (CHECKED) t = ok[dy][dz][dx] ? t : 0.0f;
+                    const float e = W.w[nidx] * prior_omega<POT>(t, inv_delta);
+                    rdd = rdd + e;
+                    rd = fmaf(e, t, rd);
+                }
+        const int iy = iy0 + r;
+        if (lane_ok && iy < g.ny) {
+            const int64_t vi = ((int64_t)iz * g.ny + iy) * g.nx + ix;
+            // capL: 0 + total, then * M (float), as the reference host does
+            const float capl = capl_scale * (0.0f + total[r]);
+            const float num = capl + beta * rd;
+            const float den = d[vi] + two_beta * rdd;
+            float xn;
+            if (guard && den == 0.0f) {
+                xn = xc;
+            } else {
+                xn = xc - num / den;
+                xn = xn < 0.0f ? 0.0f : xn;
+            }
+            st<TI>(x_out, vi, xn);
+        }
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                win[0][dz][dx] = win[1][dz][dx];
+                win[1][dz][dx] = win[2][dz][dx];
+                if (CHECKED) {
+                    ok[0][dz][dx] = ok[1][dz][dx];
+                    ok[1][dz][dx] = ok[2][dz][dx];
+                }
+            }
+    }
+}
+
 template <int POT, typename TI, typename TP>
 __global__ __launch_bounds__(256) void update_fused_kernel(
     DevGeom g, SlotTab S, int n_slots, const TP* __restrict__ hsub, const TI* __restrict__ x_in,
@@ -184,31 +286,20 @@ __global__ __launch_bounds__(256) void update_fused_kernel(
     float capl_scale, int guard) {
     const TileCoord c = tile_coord(g);
     if (c.iy0 >= g.ny) return;  // see sbp_kernel
+    const VolReader<TP> P(hsub, (uint32_t)n_slots * g.nbx * g.nby);
+    const VolReader<TI> X(x_in, (uint32_t)g.nx * g.ny * g.nz);
     float total[ROWS];
-    sbp_rows<TP>(g, S, n_slots, hsub, c.ix, c.iy0, c.iz, total);
-    if (c.ix >= g.nx) return;
-    const float two_beta = 2.0f * beta;
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        const int iy = c.iy0 + r;
-        if (iy >= g.ny) break;
-        const int64_t vi = ((int64_t)c.iz * g.ny + iy) * g.nx + c.ix;
-        // capL: 0 + total, then * M (float), as the reference host does
-        const float capl = capl_scale * (0.0f + total[r]);
-        float rdd = 0.0f, rd = 0.0f;
-        prior_voxel<POT, TI>(g, x_in, c.ix, iy, c.iz, W, inv_delta, rdd, rd);
-        const float xv = ld<TI>(x_in, vi);
-        const float num = capl + beta * rd;
-        const float den = d[vi] + two_beta * rdd;
-        float xn;
-        if (guard && den == 0.0f) {
-            xn = xv;
-        } else {
-            xn = xv - num / den;
-            xn = xn < 0.0f ? 0.0f : xn;
-        }
-        st<TI>(x_out, vi, xn);
-    }
+    sbp_rows<TP>(g, S, n_slots, P, c.ix, c.iy0, c.iz, total);
+    // wave-uniform: does the 3x3x3 neighbourhood of the wave's rows stay inside?
+    const int x_first = c.ix - (threadIdx.x & 63);
+    const bool interior = x_first >= 1 && x_first + 64 < g.nx && c.iy0 >= 1 &&
+                          c.iy0 + ROWS < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
+    if (interior)
+        update_rows<POT, false, TI>(g, X, x_out, d, c, total, W, inv_delta, beta, capl_scale,
+                                    guard);
+    else
+        update_rows<POT, true, TI>(g, X, x_out, d, c, total, W, inv_delta, beta, capl_scale,
+                                   guard);
 }
 
 // [slot][nbx][nby] -> [slot][nby][nbx], 64x64 tiles through LDS

@@ -31,7 +31,8 @@ import torch
 
 from .geometry import GAMMA_MODES, POTENTIALS, Geometry, OSTRConfig, regroup_index
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libostr_hip.so")
+LIB_PATH = os.environ.get("OSTR_HIP_LIB") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "libostr_hip.so")
 
 F32, BF16, F16 = 0, 1, 2
 _DT = {torch.float32: F32, torch.bfloat16: BF16, torch.float16: F16}
