@@ -187,6 +187,25 @@ int ostr_update_fused(ostr_handle* h, const void* x_in, void* x_out, int32_t img
  * of the subset's system matrix. Synchronous; result written to *nnz_out. */
 int ostr_count_nnz(ostr_handle* h, int32_t a0, int32_t a1, int64_t* nnz_out, void* stream);
 
+/* FP64 objective of img over ALL views (no reference counterpart; the
+ * reference has no objective). out[0] = sum_i (yhat_i + r_i) - y_i log(yhat_i
+ * + r_i) with yhat = b exp(-cos_t A x) (rays summed in f32, terms in f64;
+ * y_i = 0 contributes yhat_i + r_i); out[1] = R(x) = 1/2 sum_j sum_{k in
+ * N(j), inside} w_jk psi(x_j - x_k) in f64, psi: Q2 t^2/2, Lange
+ * d^2(|t/d| - log(1+|t/d|)), hyperbola d^2(sqrt(1+(t/d)^2) - 1). The
+ * penalized objective is out[0] + beta*out[1]. y, b, r: natural view order,
+ * reference layout [V][nbx][nby] (r may be NULL). out: host pointer to 2
+ * doubles. Synchronous; deterministic (fixed-order reductions, no atomics). */
+int ostr_objective(ostr_handle* h, const void* img, int32_t img_dtype, const float* y,
+                   const float* b, const float* r, int32_t potential, double delta,
+                   double z_xy_ratio, double* out, void* stream);
+
+/* out[0] = sum (a_i - b_i)^2, out[1] = sum b_i^2 over n elements, in f64
+ * (convergence test and error against a reference image; no reference
+ * counterpart). Synchronous; deterministic. */
+int ostr_diff_norms(ostr_handle* h, const void* a, const void* b, int32_t dtype, int64_t n,
+                    double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

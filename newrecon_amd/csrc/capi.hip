@@ -28,6 +28,8 @@ struct ostr_handle {
     unsigned long long* counter = nullptr;
     void* work = nullptr;     // grow-on-demand device work buffer
     size_t work_bytes = 0;
+    double* dwork = nullptr;  // FP64 partial sums of the reductions (+2 results)
+    size_t dwork_elems = 0;
     std::string err;
 };
 
@@ -109,6 +111,21 @@ int ensure_work(ostr_handle* h, size_t bytes, hipStream_t st, const char* fn) {
     hipError_t e = hipMalloc(&h->work, bytes);
     if (e != hipSuccess) return hip_fail(h, e, fn);
     h->work_bytes = bytes;
+    return OSTR_OK;
+}
+
+int ensure_dwork(ostr_handle* h, size_t elems, hipStream_t st, const char* fn) {
+    if (h->dwork_elems >= elems) return OSTR_OK;
+    if (h->dwork) {
+        hipError_t e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(h, e, fn);
+        (void)hipFree(h->dwork);
+        h->dwork = nullptr;
+        h->dwork_elems = 0;
+    }
+    hipError_t e = hipMalloc(&h->dwork, elems * sizeof(double));
+    if (e != hipSuccess) return hip_fail(h, e, fn);
+    h->dwork_elems = elems;
     return OSTR_OK;
 }
 
@@ -210,6 +227,7 @@ OSTR_API void ostr_destroy(ostr_handle* h) {
     if (h->views_nat_dev) (void)hipFree(h->views_nat_dev);
     if (h->counter) (void)hipFree(h->counter);
     if (h->work) (void)hipFree(h->work);
+    if (h->dwork) (void)hipFree(h->dwork);
     delete h;
 }
 
@@ -497,5 +515,66 @@ OSTR_API int ostr_count_nnz(ostr_handle* h, int32_t a0, int32_t a1, int64_t* nnz
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(h, e, "ostr_count_nnz");
     *nnz_out = (int64_t)host;
+    return OSTR_OK;
+}
+
+OSTR_API int ostr_objective(ostr_handle* h, const void* img, int32_t img_dtype, const float* y,
+                            const float* b, const float* r, int32_t potential, double delta,
+                            double z_xy_ratio, double* out, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_dtype(h, img_dtype, "ostr_objective"))) return rc;
+    if (!img || !y || !b || !out) return fail(h, OSTR_E_ARG, "ostr_objective: NULL buffer");
+    if (potential < OSTR_PRIOR_Q2 || potential > OSTR_PRIOR_HYPERBOLA)
+        return fail(h, OSTR_E_ARG, "ostr_objective: unknown potential %d", potential);
+    if (potential != OSTR_PRIOR_Q2 && delta == 0.0)
+        return fail(h, OSTR_E_ARG, "ostr_objective: delta cannot be zero");
+    if ((rc = use_device(h))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nd = ostr::objective_data_partials(h->dg, h->n_views);
+    const int64_t np = ostr::penalty_partials(h->dg);
+    if ((rc = ensure_dwork(h, (size_t)(nd + np + 2), st, "ostr_objective"))) return rc;
+    double* pd = h->dwork;
+    double* pp = h->dwork + nd;
+    double* res = h->dwork + nd + np;
+    int64_t n1 = 0, n2 = 0;
+    // all views in natural order: the objective does not depend on the OS regroup
+    hipError_t e = ostr::launch_objective_data(h->dg, h->views_nat_dev, h->n_views, img,
+                                               img_dtype, y, b, r, pd, &n1, st);
+    if (e == hipSuccess)
+        e = ostr::launch_penalty(h->dg, img, img_dtype, potential, delta, z_xy_ratio, pp, &n2, st);
+    if (e == hipSuccess) e = ostr::launch_reduce_partials(pd, n1, res, st);
+    if (e == hipSuccess) e = ostr::launch_reduce_partials(pp, n2, res + 1, st);
+    double host[2] = {0.0, 0.0};
+    if (e == hipSuccess) e = hipMemcpyAsync(host, res, sizeof host, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(h, e, "ostr_objective");
+    out[0] = host[0];
+    out[1] = 0.5 * host[1];
+    return OSTR_OK;
+}
+
+OSTR_API int ostr_diff_norms(ostr_handle* h, const void* a, const void* b, int32_t dtype,
+                             int64_t n, double* out, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_dtype(h, dtype, "ostr_diff_norms"))) return rc;
+    if (!a || !b || !out || n < 0) return fail(h, OSTR_E_ARG, "ostr_diff_norms: bad argument");
+    if ((rc = use_device(h))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t np = ostr::diff_norms_partials(n);
+    if ((rc = ensure_dwork(h, (size_t)(np + 2), st, "ostr_diff_norms"))) return rc;
+    int64_t n1 = 0;
+    hipError_t e = ostr::launch_diff_norms(a, b, dtype, n, h->dwork, &n1, st);
+    if (e == hipSuccess) e = ostr::launch_reduce_partials(h->dwork, n1 / 2, h->dwork + np, st);
+    if (e == hipSuccess)
+        e = ostr::launch_reduce_partials(h->dwork + n1 / 2, n1 / 2, h->dwork + np + 1, st);
+    double host[2] = {0.0, 0.0};
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host, h->dwork + np, sizeof host, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(h, e, "ostr_diff_norms");
+    out[0] = host[0];
+    out[1] = host[1];
     return OSTR_OK;
 }

@@ -148,4 +148,72 @@ hipError_t launch_regroup(float* dst, const float* src, int n_views, int n_subse
     return hipGetLastError();
 }
 
+namespace {
+
+constexpr int NORM_BLOCKS = 1024;
+
+// partials[k] = sum over block k's strided elements of (a - b)^2 and
+// partials[NORM_BLOCKS + k] = sum of b^2, in FP64 (fixed grid: deterministic).
+template <typename T>
+__global__ __launch_bounds__(256) void diff_norms_kernel(const T* __restrict__ a,
+                                                         const T* __restrict__ b, int64_t n,
+                                                         double* __restrict__ partials) {
+    double dd = 0.0, bb = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)NORM_BLOCKS * 256) {
+        const double bv = (double)ld<T>(b, i);
+        const double df = (double)ld<T>(a, i) - bv;
+        dd += df * df;
+        bb += bv * bv;
+    }
+    const double s0 = block_sum_256(dd);
+    __syncthreads();
+    const double s1 = block_sum_256(bb);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s0;
+        partials[NORM_BLOCKS + blockIdx.x] = s1;
+    }
+}
+
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ p,
+                                                              int64_t n,
+                                                              double* __restrict__ out) {
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) acc += p[i];
+    const double s = block_sum_256(acc);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+}  // namespace
+
+int64_t diff_norms_partials(int64_t) { return 2 * NORM_BLOCKS; }
+
+hipError_t launch_diff_norms(const void* a, const void* b, int dtype, int64_t n,
+                             double* partials, int64_t* n_partials, hipStream_t st) {
+    *n_partials = 2 * NORM_BLOCKS;
+    switch (dtype) {
+        case OSTR_F32:
+            hipLaunchKernelGGL((diff_norms_kernel<float>), dim3(NORM_BLOCKS), dim3(256), 0, st,
+                               (const float*)a, (const float*)b, n, partials);
+            break;
+        case OSTR_BF16:
+            hipLaunchKernelGGL((diff_norms_kernel<bf16_t>), dim3(NORM_BLOCKS), dim3(256), 0, st,
+                               (const bf16_t*)a, (const bf16_t*)b, n, partials);
+            break;
+        case OSTR_F16:
+            hipLaunchKernelGGL((diff_norms_kernel<_Float16>), dim3(NORM_BLOCKS), dim3(256), 0,
+                               st, (const _Float16*)a, (const _Float16*)b, n, partials);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(const double* partials, int64_t n, double* out,
+                                  hipStream_t st) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, st, partials, n, out);
+    return hipGetLastError();
+}
+
 }  // namespace ostr

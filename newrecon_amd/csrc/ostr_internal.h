@@ -96,6 +96,19 @@ __device__ __forceinline__ float VolReader<_Float16>::operator()(uint32_t i) con
     return (float)__builtin_bit_cast(_Float16, b);
 }
 
+// Sum of one double per thread over a 256-thread work-group in a fixed order
+// (wave tree, then the 4 wave sums left to right); the result is valid in
+// thread 0. Deterministic: no atomics.
+__device__ __forceinline__ double block_sum_256(double v) {
+    __shared__ double part[4];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) s = ((part[0] + part[1]) + part[2]) + part[3];
+    return s;
+}
+
 // ---- launchers (return hipError_t of the launch) -------------------------
 // Forward family. kind: 0 = projection, 1 = gamma ray length, 2 = gamma row
 // sum (reference layout out), 3 = projection + h epilogue (row-major y/b/r/h),
@@ -139,6 +152,24 @@ hipError_t launch_yry_gamma(float* gc, const float* y, const float* r, int64_t n
                             hipStream_t st);
 // Zero/constant fill by a kernel (no hipMemsetAsync, so callers may capture it).
 hipError_t launch_fill(float* p, float v, int64_t n, hipStream_t st);
+
+// FP64 objective pieces. Each writes one double per work-group into
+// `partials` (its count is returned through *n_partials); reduce_partials
+// folds them in a fixed order into *out.
+hipError_t launch_objective_data(const DevGeom& g, const ViewTab* views, int n_views,
+                                 const void* img, int img_dtype, const float* y,
+                                 const float* b, const float* r, double* partials,
+                                 int64_t* n_partials, hipStream_t st);
+hipError_t launch_penalty(const DevGeom& g, const void* img, int img_dtype, int potential,
+                          double delta, double rho, double* partials, int64_t* n_partials,
+                          hipStream_t st);
+hipError_t launch_diff_norms(const void* a, const void* b, int dtype, int64_t n,
+                             double* partials, int64_t* n_partials, hipStream_t st);
+hipError_t launch_reduce_partials(const double* partials, int64_t n, double* out,
+                                  hipStream_t st);
+int64_t objective_data_partials(const DevGeom& g, int n_views);
+int64_t penalty_partials(const DevGeom& g);
+int64_t diff_norms_partials(int64_t n);
 
 // Prior neighbour weights (host): w[27] in z,y,x order, centre 0.
 void prior_weights(float w[27], float rho);

@@ -396,4 +396,80 @@ hipError_t launch_count_nnz(const DevGeom& g, const ViewTab* views, int a0, int 
     return hipGetLastError();
 }
 
+namespace {
+
+// FP64 data term of the penalized negative Poisson log-likelihood, one term
+// per ray of every view: (yhat + r) - y log(yhat + r), yhat = b exp(-l) with
+// l = cos(t) * (Siddon sum, FP32). y, b, r in the reference layout
+// [view][nbx][nby] (natural view order). One partial sum per work-group.
+template <typename TI>
+__global__ __launch_bounds__(256) void objective_data_kernel(
+    DevGeom g, const ViewTab* __restrict__ views, int n_slots, int tiles_j,
+    const TI* __restrict__ img, const float* __restrict__ y, const float* __restrict__ b,
+    const float* __restrict__ r, double* __restrict__ partials) {
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int tj = t % tiles_j;
+    const int rest = t / tiles_j;
+    const int slot = rest % n_slots;
+    const int ti = rest / n_slots;
+    const int i = ti * TILE_I + (threadIdx.x & 63);
+    const int j = tj * TILE_J + (threadIdx.x >> 6);
+    double term = 0.0;
+    if (i < g.nbx && j < g.nby) {
+        const ViewTab v = views[slot];
+        const VolReader<TI> vol(img, (uint32_t)g.nx * g.ny * g.nz);
+        bool hit;
+        float len;
+        int nseg = 0;
+        const float s = walk<false, false, TI>(g, vol, v.x0, g.src_y, v.z0,
+                                               bin_centre(g.x_d0, i, g.dbx),
+                                               bin_centre(g.y_d0, j, g.dby), -g.det_r, hit,
+                                               len, nseg);
+        const float l = hit ? s * v.c : 0.0f;
+        const int64_t o = ((int64_t)slot * g.nbx + i) * g.nby + j;
+        const double yr = (double)b[o] * exp(-(double)l) + (r ? (double)r[o] : 0.0);
+        const double yv = (double)y[o];
+        term = yv == 0.0 ? yr : yr - yv * log(yr);
+    }
+    const double sum = block_sum_256(term);
+    if (threadIdx.x == 0) partials[blockIdx.x] = sum;
+}
+
+}  // namespace
+
+int64_t objective_data_partials(const DevGeom& g, int n_views) {
+    return (int64_t)((g.nbx + TILE_I - 1) / TILE_I) * ((g.nby + TILE_J - 1) / TILE_J) * n_views;
+}
+
+hipError_t launch_objective_data(const DevGeom& g, const ViewTab* views, int n_views,
+                                 const void* img, int img_dtype, const float* y,
+                                 const float* b, const float* r, double* partials,
+                                 int64_t* n_partials, hipStream_t st) {
+    const int tiles_j = (g.nby + TILE_J - 1) / TILE_J;
+    const int64_t nblocks = objective_data_partials(g, n_views);
+    *n_partials = nblocks;
+    if (nblocks == 0) return hipSuccess;
+    if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+    switch (img_dtype) {
+        case OSTR_F32:
+            hipLaunchKernelGGL((objective_data_kernel<float>), dim3((unsigned)nblocks), dim3(256),
+                               0, st, g, views, n_views, tiles_j, (const float*)img, y, b, r,
+                               partials);
+            break;
+        case OSTR_BF16:
+            hipLaunchKernelGGL((objective_data_kernel<bf16_t>), dim3((unsigned)nblocks),
+                               dim3(256), 0, st, g, views, n_views, tiles_j,
+                               (const bf16_t*)img, y, b, r, partials);
+            break;
+        case OSTR_F16:
+            hipLaunchKernelGGL((objective_data_kernel<_Float16>), dim3((unsigned)nblocks),
+                               dim3(256), 0, st, g, views, n_views, tiles_j,
+                               (const _Float16*)img, y, b, r, partials);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace ostr

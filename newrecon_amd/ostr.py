@@ -79,6 +79,9 @@ def lib():
         "ostr_update_fused": [vp, vp, vp, i32, vp, vp, i32, i32, i32, f32, f32, i32, f32, f32,
                               i32, vp],
         "ostr_count_nnz": [vp, i32, i32, ctypes.POINTER(i64), vp],
+        "ostr_objective": [vp, vp, i32, vp, vp, vp, i32, ctypes.c_double, ctypes.c_double,
+                           ctypes.POINTER(ctypes.c_double), vp],
+        "ostr_diff_norms": [vp, vp, vp, i32, i64, ctypes.POINTER(ctypes.c_double), vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -201,6 +204,30 @@ class DBTSystem:
         self._check(self._L.ostr_count_nnz(self._h, a0, a1, ctypes.byref(out), self._stream()))
         return int(out.value)
 
+    def objective(self, img, y, b, r=None, potential="hyperbola", delta=5e-4, z_xy_ratio=None):
+        """FP64 (data term, penalty R); the objective is data + beta * R.
+        y, b, r: natural view order, reference layout [V][nbx][nby]."""
+        rho = self.geom.dz / self.geom.dx if z_xy_ratio is None else z_xy_ratio
+        out = (ctypes.c_double * 2)()
+        self._check(self._L.ostr_objective(self._h, _p(img), _dt(img), _p(y), _p(b), _p(r),
+                                           POTENTIALS[potential], delta, rho, out,
+                                           self._stream()))
+        return float(out[0]), float(out[1])
+
+    def diff_norms(self, a, b):
+        """FP64 (sum (a-b)^2, sum b^2)."""
+        if a.dtype != b.dtype or a.numel() != b.numel():
+            raise ValueError("diff_norms: a and b must have the same dtype and size")
+        out = (ctypes.c_double * 2)()
+        self._check(self._L.ostr_diff_norms(self._h, _p(a), _p(b), _dt(a), a.numel(), out,
+                                            self._stream()))
+        return float(out[0]), float(out[1])
+
+    def rel_rmse(self, a, ref) -> float:
+        """||a - ref|| / ||ref|| in FP64."""
+        dd, bb = self.diff_norms(a, ref)
+        return (dd / bb) ** 0.5 if bb > 0 else float("inf")
+
     # -- fused --------------------------------------------------------------
     def precompute_d(self, y, r=None, gamma_mode="row_sum", d=None):
         d = self.volume() if d is None else d
@@ -318,6 +345,8 @@ class OSTR:
             raise ValueError("voxel size in x must equal voxel size in y (prior weights)")
         if self.cfg.prior != "q2" and self.cfg.delta == 0:
             raise ValueError("delta cannot be zero")
+        if not self.cfg.fused and (img_dtype != torch.float32 or prj_dtype != torch.float32):
+            raise ValueError("BF16/FP16 storage is implemented by the fused kernels only")
         self.sys = DBTSystem(geom, angles, device=device)
         self.dev = self.sys.device
         self.M, self.vs = M, nv // M
@@ -421,6 +450,14 @@ class OSTR:
 
     def run(self):
         return self.iterate(self.cfg.n_iters)
+
+    def objective(self, x=None) -> float:
+        """FP64 penalized negative log-likelihood of x (default: the current image)."""
+        cfg = self.cfg
+        data, pen = self.sys.objective(self.x if x is None else x, self.y_nat, self.b_nat,
+                                       self.r_nat, cfg.prior,
+                                       cfg.delta if cfg.prior != "q2" else 1.0, self.rho)
+        return data + cfg.beta * pen
 
 
 def save_raw(path: str, vol: torch.Tensor):

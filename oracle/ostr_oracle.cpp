@@ -439,7 +439,8 @@ inline T prior_psi(int potential, T t, T delta) {
     if (potential == OSTR_PRIOR_Q2) return t * t / 2;
     T q = std::fabs(t / delta);
     if (potential == OSTR_PRIOR_LANGE) return delta * delta * (q - std::log1p(q));
-    return delta * delta * (std::sqrt(1 + q * q) - 1);
+    // delta^2 (sqrt(1 + q^2) - 1) without the cancellation at small q
+    return delta * delta * (q * q / (std::sqrt(1 + q * q) + 1));
 }
 
 // R(x) = 1/2 sum_j sum_{k in N(j), in volume} w_jk psi(x_j - x_k)
@@ -522,7 +523,7 @@ double objective(const Geo<T>& g, const float* angles, const int* index, int n_v
     double data = 0;
     for (size_t i = 0; i < l.size(); ++i) {
         double yb = (double)b[i] * std::exp(-(double)l[i]) + (r ? (double)r[i] : 0.0);
-        data += yb - (double)y[i] * std::log(yb);
+        data += y[i] == 0 ? yb : yb - (double)y[i] * std::log(yb);
     }
     return data + (double)beta * penalty(g, x, potential, delta, rho);
 }
