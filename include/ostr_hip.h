@@ -106,6 +106,17 @@ int ostr_gamma(ostr_handle* h, float* gamma, int32_t a0, int32_t a1, int32_t mod
 int ostr_bproj_sbp(ostr_handle* h, float* img_accum, const void* prj, int32_t prj_dtype,
                    int32_t a0, int32_t a1, void* stream);
 
+/* Exact transpose of ostr_fproj, ACCUMULATED into img (f32): img_j +=
+ * sum_{a in [a0,a1)} c_a sum_i a_ij prj[a-a0][i] with c_a = cos(t_a) (cos_weight
+ * != 0) or 1, a_ij the length of ray i inside voxel j. No reference
+ * counterpart (the reference backprojects with the unmatched SBP): it makes
+ * <A x, y> = <x, A^T y>. Voxel-driven (ray/box clipping of the rays through
+ * each voxel's shadow), deterministic, no atomics. prj in the reference layout
+ * [slot][nbx][nby] (rows_layout = 0) or detector-row-major [slot][nby][nbx]. */
+int ostr_bproj_matched(ostr_handle* h, float* img_accum, const void* prj, int32_t prj_dtype,
+                       int32_t a0, int32_t a1, int32_t cos_weight, int32_t rows_layout,
+                       void* stream);
+
 /* Prior gradient and curvature ACCUMULATED into rd and rdd (f32):
  * rd_j += sum_k w_jk psi'(x_j - x_k), rdd_j += sum_k w_jk omega(x_j - x_k) over
  * the 26 neighbours, w = 1/sqrt(di^2+dj^2+(z_xy_ratio dk)^2), out-of-volume

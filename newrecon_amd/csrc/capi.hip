@@ -313,6 +313,34 @@ OSTR_API int ostr_bproj_sbp(ostr_handle* h, float* img_accum, const void* prj,
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_bproj_sbp");
 }
 
+OSTR_API int ostr_bproj_matched(ostr_handle* h, float* img_accum, const void* prj,
+                                int32_t prj_dtype, int32_t a0, int32_t a1, int32_t cos_weight,
+                                int32_t rows_layout, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_bproj_matched")) ||
+        (rc = check_dtype(h, prj_dtype, "ostr_bproj_matched")))
+        return rc;
+    if (!img_accum || !prj) return fail(h, OSTR_E_ARG, "ostr_bproj_matched: NULL buffer");
+    if ((rc = use_device(h))) return rc;
+    if (a0 == a1) return OSTR_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const void* rows = prj;
+    hipError_t e = hipSuccess;
+    if (!rows_layout) {
+        const int64_t n = (int64_t)(a1 - a0) * h->geom.nbx * h->geom.nby;
+        if ((rc = ensure_work(h, n * ostr::elem_size(prj_dtype), st, "ostr_bproj_matched")))
+            return rc;
+        e = ostr::launch_transpose_bins(prj, h->work, a1 - a0, h->geom.nbx, h->geom.nby,
+                                        prj_dtype, st);
+        rows = h->work;
+    }
+    if (e == hipSuccess)
+        e = ostr::launch_matched_rows(h->dg, h->views_host.data(), a0, a1, cos_weight, img_accum,
+                                      rows, prj_dtype, st);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_bproj_matched");
+}
+
 OSTR_API int ostr_bproj_sbp_rows(ostr_handle* h, float* img_accum, const void* prj,
                                  int32_t prj_dtype, int32_t a0, int32_t a1, void* stream) {
     CHECK_HANDLE(h);

@@ -124,6 +124,11 @@ hipError_t launch_count_nnz(const DevGeom& g, const ViewTab* views, int a0, int 
 // `views` is the host copy of the slot table.
 hipError_t launch_sbp_rows(const DevGeom& g, const ViewTab* views, int a0, int a1,
                            float* img_accum, const void* prj, int prj_dtype, hipStream_t st);
+// Exact transpose of the forward projector (voxel-driven ray/box clipping),
+// ACCUMULATED into img; prj detector-row-major.
+hipError_t launch_matched_rows(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                               int cos_weight, float* img_accum, const void* prj,
+                               int prj_dtype, hipStream_t st);
 // [slot][nbx][nby] -> [slot][nby][nbx] (dtype selects 4- or 2-byte elements).
 hipError_t launch_transpose_bins(const void* src, void* dst, int n_slots, int nbx, int nby,
                                  int dtype, hipStream_t st);

@@ -74,6 +74,7 @@ def lib():
         "ostr_regroup": [vp, vp, vp, i32, i32, vp],
         "ostr_to_rows": [vp, vp, vp, i32, i32, vp],
         "ostr_bproj_sbp_rows": [vp, vp, vp, i32, i32, i32, vp],
+        "ostr_bproj_matched": [vp, vp, vp, i32, i32, i32, i32, i32, vp],
         "ostr_precompute_d": [vp, vp, vp, vp, i32, vp],
         "ostr_fproj_h": [vp, vp, i32, vp, vp, vp, vp, i32, i32, i32, vp],
         "ostr_update_fused": [vp, vp, vp, i32, vp, vp, i32, i32, i32, f32, f32, i32, f32, f32,
@@ -188,6 +189,15 @@ class DBTSystem:
         _f32(img, "backprojection output")
         self._check(self._L.ostr_bproj_sbp(self._h, _p(img), _p(prj), _dt(prj), a0, a1,
                                            self._stream()))
+        return img
+
+    def bproj_matched(self, prj, a0, a1, img=None, cos_weight=True, rows=False):
+        """img += (cos A)^T prj: the exact transpose of fproj (prj [S][nbx][nby],
+        or detector-row-major [S][nby][nbx] with rows=True)."""
+        img = self.volume() if img is None else img
+        _f32(img, "backprojection output")
+        self._check(self._L.ostr_bproj_matched(self._h, _p(img), _p(prj), _dt(prj), a0, a1,
+                                               int(cos_weight), int(rows), self._stream()))
         return img
 
     def prior(self, img, potential="hyperbola", delta=5e-4, z_xy_ratio=None, rdd=None,

@@ -45,6 +45,7 @@ def lib():
             "orc_bproj_sbp_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i]),
             "orc_bproj_matched_f32": (_i, [_gp, _f32p, _i32p, _f32p, _f32p, _i, _i, _i]),
             "orc_bproj_matched_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i]),
+            "orc_bproj_clip_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i]),
             "orc_prior_f32": (_i, [_gp, _f32p, _f32p, _f32p, _i, _f, _f, _i]),
             "orc_prior_f64": (_i, [_gp, _f64p, _f64p, _f64p, _i, _d, _d, _i]),
             "orc_penalty_f64": (_d, [_gp, _f64p, _i, _d, _d]),
@@ -141,6 +142,15 @@ def bproj_matched(g: Geometry, angles, prj, a0, a1, index=None, cos_weight=True,
     getattr(lib(), "orc_bproj_matched_" + _dt(dtype))(
         ctypes.byref(g.to_c()), _c(angles, np.float32), _idx(index, len(angles)), img.ravel(),
         _c(prj, dtype).ravel(), a0, a1, int(cos_weight))
+    return img
+
+
+def bproj_clip(g: Geometry, angles, prj, a0, a1, index=None, cos_weight=True, img=None):
+    """img += exact geometric transpose (voxel-driven ray/box clipping), FP64."""
+    img = np.zeros(g.vol_shape, np.float64) if img is None else _c(img, np.float64).copy()
+    lib().orc_bproj_clip_f64(ctypes.byref(g.to_c()), _c(angles, np.float32),
+                             _idx(index, len(angles)), img.ravel(), _c(prj, np.float64).ravel(),
+                             a0, a1, int(cos_weight))
     return img
 
 

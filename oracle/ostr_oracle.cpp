@@ -328,6 +328,72 @@ void backproject_matched(const Geo<T>& g, const float* angles, const int* index,
     }
 }
 
+// Exact geometric transpose, voxel-driven: img_j += sum_a c_a sum_i |ray_i n
+// voxel_j| prj_i, the candidate rays being the bins whose centres lie in the
+// (floor/ceil-widened) bounding box of the voxel's corners projected from the
+// source onto the detector. The rays are the forward projector's FP32 rays
+// (source and bin centre rounded to float, Tomo_OSTR.cu:1340-1358); their
+// lengths inside each voxel are computed in T. It differs from
+// backproject_matched only where the walk's relative stop test
+// (Tomo_OSTR.cu:815) hands the last < 1e-6*max_l of a ray to the voxel before
+// the exit.
+template <typename T>
+void backproject_clip(const Geo<T>& g, const Geo<float>& gf, const float* angles,
+                      const int* index, T* img, const T* prj, int a0, int a1, int cos_weight) {
+    const T ez = (T)(-gf.det_r), sy = (T)gf.src_y;
+    for (int iz = 0; iz < g.nz; ++iz)
+        for (int iy = 0; iy < g.ny; ++iy)
+            for (int ix = 0; ix < g.nx; ++ix) {
+                const T bx0 = g.x_p0 + ix * g.dx, bx1 = g.x_p0 + (ix + 1) * g.dx;
+                const T by0 = g.y_p0 + iy * g.dy, by1 = g.y_p0 + (iy + 1) * g.dy;
+                const T bz0 = g.z_p0 + iz * g.dz, bz1 = g.z_p0 + (iz + 1) * g.dz;
+                T total = 0;
+                for (int a = a0; a < a1; ++a) {
+                    const ViewPos<float> v = view_pos(gf, angles[index[a]]);
+                    const T sx = (T)v.x0, sz = (T)v.z0;
+                    const T ta = (ez - sz) / (bz0 - sz), tb = (ez - sz) / (bz1 - sz);
+                    const T xs[4] = {sx + (bx0 - sx) * ta, sx + (bx1 - sx) * ta,
+                                     sx + (bx0 - sx) * tb, sx + (bx1 - sx) * tb};
+                    const T ys[4] = {sy + (by0 - sy) * ta, sy + (by1 - sy) * ta,
+                                     sy + (by0 - sy) * tb, sy + (by1 - sy) * tb};
+                    const T xmin = *std::min_element(xs, xs + 4), xmax = *std::max_element(xs, xs + 4);
+                    const T ymin = *std::min_element(ys, ys + 4), ymax = *std::max_element(ys, ys + 4);
+                    const int i0 = std::max(0, (int)std::floor((xmin - g.x_d0) / g.dbx - 0.5));
+                    const int i1 = std::min(g.nbx - 1, (int)std::ceil((xmax - g.x_d0) / g.dbx - 0.5));
+                    const int j0 = std::max(0, (int)std::floor((ymin - g.y_d0) / g.dby - 0.5));
+                    const int j1 = std::min(g.nby - 1, (int)std::ceil((ymax - g.y_d0) / g.dby - 0.5));
+                    T acc = 0;
+                    for (int j = j0; j <= j1; ++j)
+                        for (int i = i0; i <= i1; ++i) {
+                            const T ex = (T)cell_centre(gf.x_d0, i, gf.dbx);
+                            const T ey = (T)cell_centre(gf.y_d0, j, gf.dby);
+                            const T e[3] = {ex, ey, ez}, s[3] = {sx, sy, sz};
+                            const T lo[3] = {bx0, by0, bz0}, hi[3] = {bx1, by1, bz1};
+                            T t0 = -1e30, t1 = 1e30;
+                            for (int k = 0; k < 3; ++k) {
+                                const T d = s[k] - e[k];
+                                if (d != 0) {
+                                    const T u = (lo[k] - e[k]) / d, w = (hi[k] - e[k]) / d;
+                                    t0 = std::max(t0, std::min(u, w));
+                                    t1 = std::min(t1, std::max(u, w));
+                                } else if (!(e[k] > lo[k] && e[k] < hi[k])) {
+                                    t1 = -1;
+                                    t0 = 1;
+                                }
+                            }
+                            if (t0 < t1) {
+                                const T len = std::sqrt((ex - sx) * (ex - sx) + (ey - sy) * (ey - sy) +
+                                                        (ez - sz) * (ez - sz));
+                                acc += (t1 - t0) * len *
+                                       prj[((int64_t)(a - a0) * g.nbx + i) * g.nby + j];
+                            }
+                        }
+                    total += (cos_weight ? (T)v.c : (T)1) * acc;
+                }
+                img[((int64_t)iz * g.ny + iy) * g.nx + ix] += total;
+            }
+}
+
 // ------------------------------------------------------------- SBP ------
 template <typename T>
 void backproject_sbp(const Geo<T>& g, const float* angles, const int* index, T* img,
@@ -663,6 +729,11 @@ ORC_EXPORT int orc_bproj_sbp_f64(const ostr_geom* g, const float* ang, const int
 ORC_EXPORT int orc_bproj_matched_f32(const ostr_geom* g, const float* ang, const int* idx,
                                      float* img, const float* prj, int a0, int a1, int cw) {
     backproject_matched(Geo<float>(*g), ang, idx, img, prj, a0, a1, cw);
+    return 0;
+}
+ORC_EXPORT int orc_bproj_clip_f64(const ostr_geom* g, const float* ang, const int* idx,
+                                  double* img, const double* prj, int a0, int a1, int cw) {
+    backproject_clip(Geo<double>(*g), Geo<float>(*g), ang, idx, img, prj, a0, a1, cw);
     return 0;
 }
 ORC_EXPORT int orc_bproj_matched_f64(const ostr_geom* g, const float* ang, const int* idx,

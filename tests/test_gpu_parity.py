@@ -86,6 +86,33 @@ def test_sbp_matches_oracle(small_case, sysm, dev, oracle_lib):
     assert rel_max(got, ref) <= 1e-5
 
 
+def test_matched_backprojector_is_the_adjoint(small_case, sysm, dev, oracle_lib):
+    """<A x, y> = <x, A^T y> for the GPU pair (FP32), and A^T agrees with the
+    oracle's exact geometric transpose (voxel-driven clip, FP64)."""
+    s, x, _, _ = small_case
+    rng = np.random.default_rng(11)
+    yv = rng.standard_normal((s.n_views,) + s.geom.view_shape).astype(np.float32)
+    xv = rng.random(s.geom.vol_shape).astype(np.float32)
+    Ax = sysm.fproj(T(xv, dev), 0, s.n_views).cpu().numpy().astype(np.float64)
+    Aty = sysm.bproj_matched(T(yv, dev), 0, s.n_views).cpu().numpy()
+    lhs = float((Ax * yv).sum())
+    rhs = float((xv.astype(np.float64) * Aty).sum())
+    # y is zero-mean, so the sums cancel heavily: bound the FP32 difference by
+    # the magnitude of the terms, sum |Ax| |y|
+    scale = float((np.abs(Ax) * np.abs(yv)).sum())
+    assert abs(lhs - rhs) <= 1e-7 * scale, (lhs, rhs, scale)
+    ref = oracle_lib.bproj_clip(s.geom, s.angles, yv.astype(np.float64), 0, s.n_views)
+    assert rel_max(Aty, ref) <= 1e-5
+    # row-major input gives the same result; subsets through the index table too
+    rows = sysm.bproj_matched(T(yv.transpose(0, 2, 1), dev), 0, s.n_views, rows=True)
+    assert np.array_equal(rows.cpu().numpy(), Aty)
+    idx = regroup_index(s.n_views, 3)
+    sysr = DBTSystem(s.geom, s.angles, index=idx, device=dev)
+    got = sysr.bproj_matched(T(yv[:3], dev), 3, 6).cpu().numpy()
+    ref = oracle_lib.bproj_clip(s.geom, s.angles, yv[:3].astype(np.float64), 3, 6, index=idx)
+    assert rel_max(got, ref) <= 1e-5
+
+
 def test_row_major_layout_entry_points(small_case, sysm, dev, oracle_lib):
     """to_rows is an exact transpose; SBP of the row-major copy equals SBP of
     the reference layout; regroup(to_rows) = transpose of regroup."""
