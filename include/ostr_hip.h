@@ -117,6 +117,16 @@ int ostr_bproj_matched(ostr_handle* h, float* img_accum, const void* prj, int32_
                        int32_t a0, int32_t a1, int32_t cos_weight, int32_t rows_layout,
                        void* stream);
 
+/* One-step backprojected image of ReconDBT/BP.m:54-82, written to img (f32),
+ * with G = cos(t) A: per view l = G 1, q = p / l where l > |dz|/10 (else 0),
+ * img += (G^T q) / (G^T 1) (0 where G^T 1 = 0), then img / n_views. The
+ * reference's OSTR starts from zeros; this is the start the warm-start path
+ * refines. p_or_y: line integrals p (counts = 0) or counts y with p = log(b /
+ * max(y, 1)) (counts != 0), natural view order, reference layout
+ * [V][nbx][nby]; b only with counts. Uses the handle's work buffer. */
+int ostr_bp_init(ostr_handle* h, float* img, const float* p_or_y, const float* b,
+                 int32_t counts, void* stream);
+
 /* Prior gradient and curvature ACCUMULATED into rd and rdd (f32):
  * rd_j += sum_k w_jk psi'(x_j - x_k), rdd_j += sum_k w_jk omega(x_j - x_k) over
  * the 26 neighbours, w = 1/sqrt(di^2+dj^2+(z_xy_ratio dk)^2), out-of-volume

@@ -341,6 +341,46 @@ OSTR_API int ostr_bproj_matched(ostr_handle* h, float* img_accum, const void* pr
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_bproj_matched");
 }
 
+OSTR_API int ostr_bp_init(ostr_handle* h, float* img, const float* p_or_y, const float* b,
+                          int32_t counts, void* stream) {
+    CHECK_HANDLE(h);
+    if (!img || !p_or_y || (counts && !b))
+        return fail(h, OSTR_E_ARG, "ostr_bp_init: NULL buffer");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const ostr_geom& G = h->geom;
+    const int64_t nbin = (int64_t)G.nbx * G.nby, nvox = (int64_t)G.nx * G.ny * G.nz;
+    if ((rc = ensure_work(h, (size_t)(3 * nbin + 2 * nvox) * sizeof(float), st, "ostr_bp_init")))
+        return rc;
+    float* l = (float*)h->work;
+    float* q = l + nbin;
+    float* ones = q + nbin;
+    float* num = ones + nbin;
+    float* den = num + nvox;
+    const float thr = std::fabs(G.dz) / 10.0f;
+    hipError_t e = ostr::launch_fill(img, 0.0f, nvox, st);
+    if (e == hipSuccess) e = ostr::launch_fill(ones, 1.0f, nbin, st);
+    // natural view order, identity table (BP.m loops over the views as acquired)
+    for (int v = 0; v < h->n_views && e == hipSuccess; ++v) {
+        e = ostr::launch_project(h->dg, h->views_nat_dev, v, v + 1, 2, 0, nullptr, OSTR_F32, l,
+                                 OSTR_F32, nullptr, nullptr, nullptr, st);
+        if (e == hipSuccess)
+            e = ostr::launch_bp_data(q, l, p_or_y + v * nbin, counts ? b + v * nbin : nullptr,
+                                     counts, h->views_nat[v].c, thr, G.nbx, G.nby, st);
+        if (e == hipSuccess) e = ostr::launch_fill(num, 0.0f, 2 * nvox, st);
+        if (e == hipSuccess)
+            e = ostr::launch_matched_rows(h->dg, h->views_nat.data(), v, v + 1, 0, num, q,
+                                          OSTR_F32, st);
+        if (e == hipSuccess)
+            e = ostr::launch_matched_rows(h->dg, h->views_nat.data(), v, v + 1, 0, den, ones,
+                                          OSTR_F32, st);
+        if (e == hipSuccess) e = ostr::launch_bp_accum(img, num, den, nvox, st);
+    }
+    if (e == hipSuccess) e = ostr::launch_div(img, (float)h->n_views, nvox, st);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_bp_init");
+}
+
 OSTR_API int ostr_bproj_sbp_rows(ostr_handle* h, float* img_accum, const void* prj,
                                  int32_t prj_dtype, int32_t a0, int32_t a1, void* stream) {
     CHECK_HANDLE(h);

@@ -94,6 +94,40 @@ __global__ __launch_bounds__(256) void regroup_kernel(float* __restrict__ dst,
         o[i] = s[i];
 }
 
+// BP.m step for one view: q = p / (c l) where c l > thr, else 0. p is the line
+// integral (counts = 0) or log(b / max(y, 1)) of counts y and blank b; l and
+// p/y/b in the reference layout [nbx][nby], q written detector-row-major.
+__global__ __launch_bounds__(256) void bp_data_kernel(float* __restrict__ q_rows,
+                                                      const float* __restrict__ l,
+                                                      const float* __restrict__ py,
+                                                      const float* __restrict__ b, int counts,
+                                                      float c, float thr, int nbx, int nby) {
+    const int64_t n = (int64_t)nbx * nby;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * 256) {
+        const int i = (int)(k / nby), j = (int)(k % nby);
+        const float lv = c * l[k];
+        float p = py[k];
+        if (counts) p = logf(b[k] / fmaxf(p, 1.0f));
+        q_rows[(int64_t)j * nbx + i] = lv > thr ? p / lv : 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(256) void bp_accum_kernel(float* __restrict__ img,
+                                                       const float* __restrict__ num,
+                                                       const float* __restrict__ den,
+                                                       int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        if (den[i] != 0.0f) img[i] = img[i] + num[i] / den[i];
+}
+
+__global__ __launch_bounds__(256) void div_kernel(float* __restrict__ p, float v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        p[i] = p[i] / v;
+}
+
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, float v, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * 256)
@@ -101,6 +135,24 @@ __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, float 
 }
 
 }  // namespace
+
+hipError_t launch_bp_data(float* q_rows, const float* l, const float* py, const float* b,
+                          int counts, float c, float thr, int nbx, int nby, hipStream_t st) {
+    hipLaunchKernelGGL(bp_data_kernel, dim3(grid_for((int64_t)nbx * nby)), dim3(256), 0, st,
+                       q_rows, l, py, b, counts, c, thr, nbx, nby);
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_accum(float* img, const float* num, const float* den, int64_t n,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(bp_accum_kernel, dim3(grid_for(n)), dim3(256), 0, st, img, num, den, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_div(float* p, float v, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(div_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill(float* p, float v, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;

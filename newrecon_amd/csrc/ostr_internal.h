@@ -155,6 +155,12 @@ hipError_t launch_regroup(float* dst, const float* src, int n_views, int n_subse
                           int64_t nbin, hipStream_t st);
 hipError_t launch_yry_gamma(float* gc, const float* y, const float* r, int64_t n,
                             hipStream_t st);
+// BP.m initial image pieces (one view at a time).
+hipError_t launch_bp_data(float* q_rows, const float* l, const float* py, const float* b,
+                          int counts, float c, float thr, int nbx, int nby, hipStream_t st);
+hipError_t launch_bp_accum(float* img, const float* num, const float* den, int64_t n,
+                           hipStream_t st);
+hipError_t launch_div(float* p, float v, int64_t n, hipStream_t st);
 // Zero/constant fill by a kernel (no hipMemsetAsync, so callers may capture it).
 hipError_t launch_fill(float* p, float v, int64_t n, hipStream_t st);
 

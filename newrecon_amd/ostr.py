@@ -75,6 +75,7 @@ def lib():
         "ostr_to_rows": [vp, vp, vp, i32, i32, vp],
         "ostr_bproj_sbp_rows": [vp, vp, vp, i32, i32, i32, vp],
         "ostr_bproj_matched": [vp, vp, vp, i32, i32, i32, i32, i32, vp],
+        "ostr_bp_init": [vp, vp, vp, vp, i32, vp],
         "ostr_precompute_d": [vp, vp, vp, vp, i32, vp],
         "ostr_fproj_h": [vp, vp, i32, vp, vp, vp, vp, i32, i32, i32, vp],
         "ostr_update_fused": [vp, vp, vp, i32, vp, vp, i32, i32, i32, f32, f32, i32, f32, f32,
@@ -198,6 +199,19 @@ class DBTSystem:
         _f32(img, "backprojection output")
         self._check(self._L.ostr_bproj_matched(self._h, _p(img), _p(prj), _dt(prj), a0, a1,
                                                int(cos_weight), int(rows), self._stream()))
+        return img
+
+    def bp_init(self, p=None, y=None, b=None, img=None):
+        """One-step backprojected image (ReconDBT/BP.m:54-82) of line integrals
+        p, or of counts y with blank b (p = log(b / max(y, 1))); natural view
+        order, reference layout [V][nbx][nby]."""
+        img = self.volume() if img is None else img
+        counts = p is None
+        src = y if counts else p
+        if src is None or (counts and b is None):
+            raise ValueError("bp_init needs p, or y and b")
+        self._check(self._L.ostr_bp_init(self._h, _p(img), _p(src), _p(b) if counts else None,
+                                         int(counts), self._stream()))
         return img
 
     def prior(self, img, potential="hyperbola", delta=5e-4, z_xy_ratio=None, rdd=None,
@@ -370,8 +384,12 @@ class OSTR:
             return t.to(device=self.dev, dtype=torch.float32).contiguous()
 
         self.y_nat, self.b_nat, self.r_nat = dev32(y), dev32(b), dev32(r)
-        self.x = (self.sys.volume(img_dtype) if x0 is None
-                  else dev32(x0).to(img_dtype).contiguous())
+        if x0 is None:
+            self.x = self.sys.volume(img_dtype)
+        elif isinstance(x0, str) and x0 == "bp":
+            self.x = self.sys.bp_init(y=self.y_nat, b=self.b_nat).to(img_dtype).contiguous()
+        else:
+            self.x = dev32(x0).to(img_dtype).contiguous()
         self.iteration = 0
         self._precomputed = False
         # optional per-kernel timing: name -> list of (start, end) torch.cuda.Events

@@ -46,6 +46,7 @@ def lib():
             "orc_bproj_matched_f32": (_i, [_gp, _f32p, _i32p, _f32p, _f32p, _i, _i, _i]),
             "orc_bproj_matched_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i]),
             "orc_bproj_clip_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i]),
+            "orc_bp_init_f64": (_i, [_gp, _f32p, _i, _f64p, _f64p]),
             "orc_prior_f32": (_i, [_gp, _f32p, _f32p, _f32p, _i, _f, _f, _i]),
             "orc_prior_f64": (_i, [_gp, _f64p, _f64p, _f64p, _i, _d, _d, _i]),
             "orc_penalty_f64": (_d, [_gp, _f64p, _i, _d, _d]),
@@ -151,6 +152,14 @@ def bproj_clip(g: Geometry, angles, prj, a0, a1, index=None, cos_weight=True, im
     lib().orc_bproj_clip_f64(ctypes.byref(g.to_c()), _c(angles, np.float32),
                              _idx(index, len(angles)), img.ravel(), _c(prj, np.float64).ravel(),
                              a0, a1, int(cos_weight))
+    return img
+
+
+def bp_init(g: Geometry, angles, p):
+    """One-step backprojected image (ReconDBT/BP.m:54-82) of line integrals p, FP64."""
+    img = np.zeros(g.vol_shape, np.float64)
+    lib().orc_bp_init_f64(ctypes.byref(g.to_c()), _c(angles, np.float32), len(angles),
+                          _c(p, np.float64).ravel(), img.ravel())
     return img
 
 

@@ -394,6 +394,37 @@ void backproject_clip(const Geo<T>& g, const Geo<float>& gf, const float* angles
             }
 }
 
+// One-step backprojected image of ReconDBT/BP.m:54-82 with G = cos(t) A (the
+// OSTR forward model): per view, l = G 1 (row sums), q = p / l where
+// l > |dz|/10 (else 0), img += (G^T q) / (G^T 1) (0 where G^T 1 = 0); finally
+// img / n_views. p: line integrals [V][nbx][nby], natural view order. The
+// per-view cos cancels in G^T q / G^T 1, so both use the plain transpose.
+template <typename T>
+void bp_init(const Geo<T>& g, const Geo<float>& gf, const float* angles, int n_views,
+             const T* p, T* img) {
+    const int64_t nbin = g.nbin(), nvox = g.nvox();
+    std::vector<int> idx(n_views);
+    for (int i = 0; i < n_views; ++i) idx[i] = i;
+    std::vector<T> l(nbin), q(nbin), ones(nbin, (T)1), num(nvox), den(nvox);
+    std::fill(img, img + nvox, (T)0);
+    const T thr = std::fabs(g.dz) / 10;
+    for (int v = 0; v < n_views; ++v) {
+        project(g, angles, idx.data(), (const T*)nullptr, l.data(), v, v + 1, 2, 0, 0);
+        const T c = (T)view_pos(gf, angles[v]).c;
+        for (int64_t i = 0; i < nbin; ++i) {
+            const T lv = c * l[i];
+            q[i] = lv > thr ? p[(int64_t)v * nbin + i] / lv : (T)0;
+        }
+        std::fill(num.begin(), num.end(), (T)0);
+        std::fill(den.begin(), den.end(), (T)0);
+        backproject_clip(g, gf, angles, idx.data(), num.data(), q.data(), v, v + 1, 0);
+        backproject_clip(g, gf, angles, idx.data(), den.data(), ones.data(), v, v + 1, 0);
+        for (int64_t j = 0; j < nvox; ++j)
+            if (den[j] != 0) img[j] += num[j] / den[j];
+    }
+    for (int64_t j = 0; j < nvox; ++j) img[j] /= (T)n_views;
+}
+
 // ------------------------------------------------------------- SBP ------
 template <typename T>
 void backproject_sbp(const Geo<T>& g, const float* angles, const int* index, T* img,
@@ -729,6 +760,11 @@ ORC_EXPORT int orc_bproj_sbp_f64(const ostr_geom* g, const float* ang, const int
 ORC_EXPORT int orc_bproj_matched_f32(const ostr_geom* g, const float* ang, const int* idx,
                                      float* img, const float* prj, int a0, int a1, int cw) {
     backproject_matched(Geo<float>(*g), ang, idx, img, prj, a0, a1, cw);
+    return 0;
+}
+ORC_EXPORT int orc_bp_init_f64(const ostr_geom* g, const float* ang, int nv, const double* p,
+                               double* img) {
+    bp_init(Geo<double>(*g), Geo<float>(*g), ang, nv, p, img);
     return 0;
 }
 ORC_EXPORT int orc_bproj_clip_f64(const ostr_geom* g, const float* ang, const int* idx,

@@ -238,6 +238,23 @@ def test_clip_transpose_vs_walk_transpose(small_case, oracle_lib):
     assert diff <= 1e-6 * 660 * np.abs(y).max() * 4 * s.n_views
 
 
+def test_bp_init_constant_volume(oracle_lib):
+    """BP.m's one-step image of the exact line integrals of a uniform volume
+    is that uniform value wherever every ray through a voxel is unmasked."""
+    # bins (0.7 mm) finer than voxels (2 mm): every voxel is crossed by rays
+    g = tiny_geom(nx=6, ny=5, nz=4, nbx=60, nby=40, db=0.7)
+    ang = arc_angles(3, 7.5)
+    c = 0.0456
+    p = oracle_lib.fproj(g, ang, np.full(g.vol_shape, c), 0, 3, dtype=np.float64)
+    img = oracle_lib.bp_init(g, ang, p)
+    # p comes from FP64 rays, the transpose uses the projector's float rays and
+    # float cos: agreement to ~1e-8
+    np.testing.assert_allclose(img[:, 1:-1, 1:-1], c, rtol=1e-6)
+    # and it is linear in p
+    img2 = oracle_lib.bp_init(g, ang, 2.5 * p)
+    np.testing.assert_allclose(img2, 2.5 * img, rtol=1e-12, atol=1e-15)
+
+
 def sbp_textbook(g, ang, prj):
     """SBP via scipy bilinear interpolation at the projected voxel centres."""
     out = np.zeros(g.vol_shape)
