@@ -85,12 +85,108 @@ def cpu_baseline(scan, x_host, y_host, b_host, threads):
     }
 
 
+def bench_batch(args, dev):
+    """Uneven batch: 8 cases (30..80 slices, 800x400..1400x700 voxels, 9/15/25
+    views, 3/5/5 subsets), FP32 images, BF16 subset sinograms, tile shapes from
+    the autotuner. A step is one batched subset of every case (two launches).
+    Reports aggregate and per-case throughput, and the same cases run one
+    after another with the single-case kernels."""
+    import torch
+
+    from newrecon_amd.autotune import tune
+    from newrecon_amd.batch import BatchOSTR
+    from newrecon_amd.geometry import scan_batch
+    from newrecon_amd.ostr import OSTR
+    from newrecon_amd.workload import make_measurement
+
+    scans = scan_batch()
+    recs = []
+    for s in scans:
+        print(f"[bench] case {s.name}: phantom and data", file=sys.stderr, flush=True)
+        _, y, b = make_measurement(s, device=dev)
+        cfg = s.recon
+        cfg.fused = True
+        recs.append(OSTR(s.geom, s.angles, y, b, cfg=cfg, prj_dtype=torch.bfloat16,
+                         device=dev))
+        del y, b
+    bt = BatchOSTR(recs)
+    torch.cuda.synchronize()
+    tuned = tune(bt, cache_path=args.tune_cache, reps=3)
+    print(f"[bench] tiles {bt.tile.as_dict()} (cached={tuned['cached']})", file=sys.stderr,
+          flush=True)
+    bt.step(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bt.step(args.steps)
+    torch.cuda.synchronize()
+    s_step = (time.perf_counter() - t0) / args.steps
+
+    # kernel durations (events on the launch stream)
+    def timed(fn, n=5):
+        e = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+             for _ in range(n)]
+        for a, z in e:
+            a.record()
+            fn()
+            z.record()
+        torch.cuda.synchronize()
+        return sum(a.elapsed_time(z) for a, z in e) / n
+
+    k_fwd = timed(bt.fproj_h)
+    k_upd = timed(bt.update)
+    # the same subsets, case by case, with the single-case fused kernels
+    alone = []
+    for k, rec in enumerate(recs):
+        m = bt.pos[k]
+        alone.append(timed(lambda: rec._subset(m)))
+    updates, bytes_fwd, bytes_upd, per_case = 0, 0, 0, []
+    for s, rec, t_alone in zip(scans, recs, alone):
+        g = s.geom
+        nnz = rec.sys.count_nnz(0, s.n_views) / rec.M   # per subset, on average
+        upd = nnz + g.n_vox * rec.vs
+        updates += upd
+        bytes_fwd += algorithmic_bytes(g, rec.vs, "fproj_h", 4, 2)
+        bytes_upd += algorithmic_bytes(g, rec.vs, "update_fused", 4, 2)
+        per_case.append({"case": s.name, "subsets": rec.M, "views": s.n_views,
+                         "voxel_ray_updates_per_subset": int(upd),
+                         "alone_ms_per_subset": round(t_alone, 4),
+                         "alone_s_per_iter": round(t_alone * rec.M / 1e3, 6)})
+    dominant, k_dom, b_dom = (("batch_update_fused", k_upd, bytes_upd) if k_upd >= k_fwd
+                              else ("batch_fproj_h", k_fwd, bytes_fwd))
+    achieved = b_dom / (k_dom * 1e-3) / 1e9
+    out = {
+        "metric": "batched OSTR voxel-ray updates/s (8 uneven cases, one launch per kernel)",
+        "value": round(updates / s_step, 1),
+        "unit": "voxel-ray updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(s_step * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 image, bf16 subset sinogram (h)",
+        "data": "synthetic (seeded breast phantoms, Poisson counts)",
+        "config": {"workload": "dbt-batch-8-uneven", "cases": [s.name for s in scans],
+                   "tiles": bt.tile.as_dict(), "tuner_cached": tuned["cached"]},
+        "kernel_ms": {"batch_fproj_h": round(k_fwd, 4), "batch_update_fused": round(k_upd, 4)},
+        "sequential_single_case_ms_per_step": round(sum(alone), 4),
+        "batch_speedup_vs_sequential": round(sum(alone) / (s_step * 1e3), 3),
+        "per_case": per_case,
+        "roofline": {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": b_dom},
+        "tuner_ms": {k: tuned.get(k) for k in ("fwd_ms", "upd_ms")},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="clinical", choices=["clinical", "small"])
+    ap.add_argument("--config", default="clinical", choices=["clinical", "small", "batch"])
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache (JSON) for --config batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -111,6 +207,8 @@ def main(argv=None):
     from newrecon_amd.workload import make_measurement
 
     dev = torch.device("cuda:0")
+    if args.config == "batch":
+        return bench_batch(args, dev)
     scan = SCANS[args.config]()
     g, cfg = scan.geom, scan.recon
     cfg.fused = True

@@ -227,6 +227,60 @@ int ostr_objective(ostr_handle* h, const void* img, int32_t img_dtype, const flo
 int ostr_diff_norms(ostr_handle* h, const void* a, const void* b, int32_t dtype, int64_t n,
                     double* out, void* stream);
 
+/* ---- batched cases: one launch over several reconstructions ----------------
+ * The reference runs one geometry per process (__constant__ state,
+ * Tomo_OSTR.cu:81-115) and so one case per launch; these calls have no
+ * reference counterpart. They run the two fused OSTR kernels (ostr_fproj_h and
+ * ostr_update_fused) for the current subset of every case of a batch in ONE
+ * launch each; cases may differ in geometry, thickness and view count. Per
+ * case the results are bit-identical to the single-case calls (every bin and
+ * voxel is computed by the same code in the same order; the tile shape only
+ * changes which work-group computes it). Images are f32. */
+typedef struct ostr_batch ostr_batch;
+
+typedef struct ostr_batch_case {
+    ostr_handle* h;         /* geometry and slot table of the case             */
+    int32_t a0, a1;         /* slots [a0, a1) of the handle's index (<= 64)   */
+    const void* x_in;       /* image read (f32)                                */
+    void* x_out;            /* image written by ostr_batch_update_fused        */
+    const float* d;         /* denominator [nz][ny][nx]                        */
+    const float* y;         /* subset counts, detector-row-major [slot][nby][nbx] */
+    const float* b;         /* subset blank, same layout                       */
+    const float* r;         /* subset scatter, same layout, or NULL            */
+    void* hsub;             /* subset h (prj dtype), written by ostr_batch_fproj_h */
+    float beta, capl_scale, delta, z_xy_ratio;
+    int32_t guard;
+} ostr_batch_case;
+
+/* Tile and grid shapes of the batched launches (the autotuner's choices).
+ * groups_per_cu = 0: one work-group per tile; k > 0: a persistent grid of k
+ * work-groups per compute unit, each looping over its share of the tiles. */
+typedef struct ostr_tile_cfg {
+    int32_t fwd_rows;           /* detector rows per forward work-group: 1, 2, 4, 8 */
+    int32_t fwd_groups_per_cu;
+    int32_t upd_rows;           /* voxel rows per wave in the update: 4 or 8        */
+    int32_t upd_waves;          /* waves per update work-group: (8,2) (8,4) (4,4)   */
+    int32_t upd_groups_per_cu;
+} ostr_tile_cfg;
+
+/* A batch object on `device` for up to max_cases cases: owns the device case
+ * table and its pinned staging ring. Calls on one batch object are ordered by
+ * the caller (one host thread). */
+int ostr_batch_create(int32_t device, int32_t max_cases, ostr_batch** out);
+void ostr_batch_destroy(ostr_batch* bt);
+const char* ostr_batch_last_error(const ostr_batch* bt);
+
+/* ostr_fproj_h for every case: hsub_c = h(y_c, b_c, r_c, cos_t * A_c x_in_c)
+ * over the slots [a0_c, a1_c) of case c. */
+int ostr_batch_fproj_h(ostr_batch* bt, const ostr_batch_case* cases, int32_t n_cases,
+                       int32_t prj_dtype, const ostr_tile_cfg* cfg, void* stream);
+
+/* ostr_update_fused for every case (one prior potential for the batch; beta,
+ * delta, z_xy_ratio, capl_scale, guard per case). */
+int ostr_batch_update_fused(ostr_batch* bt, const ostr_batch_case* cases, int32_t n_cases,
+                            int32_t prj_dtype, int32_t potential, const ostr_tile_cfg* cfg,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,15 +33,12 @@ namespace ostr {
 
 namespace {
 
-constexpr int MAX_SLOTS = 64;  // views per launch handled in one pass
-constexpr int ROWS = 8;        // voxel rows (y) per wave
-constexpr int WAVES = 4;       // waves per work-group
-constexpr int TILE_X = 64, TILE_Y = ROWS * WAVES;
-
-struct SlotTab {
-    float x0[MAX_SLOTS], z0[MAX_SLOTS];
-    float num[MAX_SLOTS];  // |S|^2 evaluated as fma(z0,z0,fma(y0,y0,x0*x0))
-};
+// Tile shape of the single-case kernels: CASE_ROWS voxel rows (y) per wave,
+// CASE_WAVES waves per work-group, so a work-group covers 64 x 32 voxels of
+// one slice. The batched kernels take the shape as template arguments (R, NW).
+constexpr int CASE_ROWS = 8;
+constexpr int CASE_WAVES = 4;
+constexpr int TILE_X = 64;
 
 __device__ __forceinline__ float cell_centre(float origin, int i, float size) {
     return (float)((double)origin + ((double)i + 0.5) * (double)size);
@@ -58,15 +55,18 @@ struct TileCoord {
     int ix, iy0, iz;
 };
 
-__device__ __forceinline__ TileCoord tile_coord(const DevGeom& g) {
+// Voxel coordinates of this thread in volume tile t (x tiles fastest, then y
+// tiles, then slices); the wave owns rows iy0 .. iy0+R-1.
+template <int R, int NW>
+__device__ __forceinline__ TileCoord tile_coord(const DevGeom& g, int t) {
+    constexpr int TILE_Y = R * NW;
     const int tiles_x = (g.nx + TILE_X - 1) / TILE_X;
     const int tiles_y = (g.ny + TILE_Y - 1) / TILE_Y;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int bx = t % tiles_x;
     const int rest = t / tiles_x;
     TileCoord c;
     c.ix = bx * TILE_X + (threadIdx.x & 63);
-    c.iy0 = (rest % tiles_y) * TILE_Y + (threadIdx.x >> 6) * ROWS;
+    c.iy0 = (rest % tiles_y) * TILE_Y + (threadIdx.x >> 6) * R;
     c.iz = rest / tiles_y;
     return c;
 }
@@ -84,7 +84,7 @@ __device__ __forceinline__ float readlane_f(float v, int lane) {
 // row's two tap rows lie on the detector (the common case), the four taps are
 // loaded without masks; otherwise each tap is masked on its own, as the
 // reference does.
-template <typename TP>
+template <typename TP, int ROWS>
 __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int n,
                                          const VolReader<TP>& P, int ix, int iy0, int iz,
                                          float total[ROWS]) {
@@ -95,6 +95,11 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
     const float y0 = g.src_y;
     const float x1 = cell_centre(g.x_p0, ix, g.dx);
     const float z1 = cell_centre(g.z_p0, iz, g.dz);
+    // slot parameters: lane a holds slot a (n <= 64); the loops below take
+    // them with readlane (uniform) or shfl (per lane), no memory reads
+    const float sx0 = lane < n ? S.x0[lane] : 0.0f;
+    const float sz0 = lane < n ? S.z0[lane] : 0.0f;
+    const float snum = lane < n ? S.num[lane] : 0.0f;
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) total[r] = 0.0f;
     for (int c0 = 0; c0 < n; c0 += VIEW_CHUNK) {
@@ -102,8 +107,8 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
         float rt_fj = 0.0f, rt_ey = 0.0f;
         {
             const int r = lane % ROWS, a = c0 + lane / ROWS;
+            const float z0 = __shfl(sz0, a & 63, 64);
             if (a < n) {
-                const float z0 = S.z0[a];
                 const float y1 = cell_centre(g.y_p0, iy0 + r, g.dy);
                 rt_ey = y0 - y1;
                 if (z1 != z0) {
@@ -116,7 +121,8 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
         const int na = n - c0 < VIEW_CHUNK ? n - c0 : VIEW_CHUNK;
         for (int k = 0; k < na; ++k) {
             const int a = c0 + k;
-            const float x0 = S.x0[a], z0 = S.z0[a], num = S.num[a];
+            const float x0 = readlane_f(sx0, a), z0 = readlane_f(sz0, a);
+            const float num = readlane_f(snum, a);
             if (z1 == z0) continue;
             const float ex = x0 - x1, ez = z0 - z1;
             const float exx = ex * ex;
@@ -169,16 +175,16 @@ template <typename TP>
 __global__ __launch_bounds__(256) void sbp_kernel(DevGeom g, SlotTab S, int n_slots,
                                                   const TP* __restrict__ prj,
                                                   float* __restrict__ img) {
-    const TileCoord c = tile_coord(g);
+    const TileCoord c = tile_coord<CASE_ROWS, CASE_WAVES>(g, xcd_tile(blockIdx.x, gridDim.x));
     // whole waves past the last row leave; lanes past the last column stay
     // (they evaluate row terms other lanes read) and only skip the stores
     if (c.iy0 >= g.ny) return;
     const VolReader<TP> P(prj, (uint32_t)n_slots * g.nbx * g.nby);
-    float total[ROWS];
-    sbp_rows<TP>(g, S, n_slots, P, c.ix, c.iy0, c.iz, total);
+    float total[CASE_ROWS];
+    sbp_rows<TP, CASE_ROWS>(g, S, n_slots, P, c.ix, c.iy0, c.iz, total);
     if (c.ix >= g.nx) return;
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
+    for (int r = 0; r < CASE_ROWS; ++r) {
         const int iy = c.iy0 + r;
         if (iy >= g.ny) break;
         const int64_t vi = ((int64_t)c.iz * g.ny + iy) * g.nx + c.ix;
@@ -193,13 +199,12 @@ __global__ __launch_bounds__(256) void sbp_kernel(DevGeom g, SlotTab S, int n_sl
 // dx inner), then the voxel is updated and stored at once. CHECKED: the tile
 // touches the volume boundary; a neighbour outside the volume contributes
 // t = 0 (it is replaced by the voxel itself, Tomo_OSTR.cu:238-241).
-template <int POT, bool CHECKED, typename TI>
+template <int POT, bool CHECKED, typename TI, int ROWS>
 __device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI>& X,
-                                            TI* __restrict__ x_out,
-                                            const float* __restrict__ d, const TileCoord& c,
-                                            const float total[ROWS], const PriorW& W,
-                                            float inv_delta, float beta, float capl_scale,
-                                            int guard) {
+                                            const VolWriter<TI>& XO, const VolReader<float>& D,
+                                            const TileCoord& c, const float total[ROWS],
+                                            const PriorClass& W, float inv_delta, float beta,
+                                            float capl_scale, int guard) {
     const int ix = c.ix, iy0 = c.iy0, iz = c.iz;
     const uint32_t nx = g.nx, plane = (uint32_t)g.nx * g.ny;
     const float two_beta = 2.0f * beta;
@@ -245,17 +250,17 @@ __device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI
                     if (nidx == 13) continue;
                     float t = xc - win[dy][dz][dx];
                     if (CHECKED) t = ok[dy][dz][dx] ? t : 0.0f;
-                    const float e = W.w[nidx] * prior_omega<POT>(t, inv_delta);
+                    const float e = W.w[prior_class(nidx)] * prior_omega<POT>(t, inv_delta);
                     rdd = rdd + e;
                     rd = fmaf(e, t, rd);
                 }
         const int iy = iy0 + r;
         if (lane_ok && iy < g.ny) {
-            const int64_t vi = ((int64_t)iz * g.ny + iy) * g.nx + ix;
+            const uint32_t vi = (uint32_t)iz * plane + (uint32_t)iy * nx + (uint32_t)ix;
             // capL: 0 + total, then * M (float), as the reference host does
             const float capl = capl_scale * (0.0f + total[r]);
             const float num = capl + beta * rd;
-            const float den = d[vi] + two_beta * rdd;
+            const float den = D(vi) + two_beta * rdd;
             float xn;
             if (guard && den == 0.0f) {
                 xn = xc;
@@ -263,7 +268,7 @@ __device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI
                 xn = xc - num / den;
                 xn = xn < 0.0f ? 0.0f : xn;
             }
-            st<TI>(x_out, vi, xn);
+            XO(vi, xn);
         }
 #pragma unroll
         for (int dz = 0; dz < 3; ++dz)
@@ -279,27 +284,71 @@ __device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI
     }
 }
 
-template <int POT, typename TI, typename TP>
-__global__ __launch_bounds__(256) void update_fused_kernel(
-    DevGeom g, SlotTab S, int n_slots, const TP* __restrict__ hsub, const TI* __restrict__ x_in,
-    TI* __restrict__ x_out, const float* __restrict__ d, PriorW W, float inv_delta, float beta,
-    float capl_scale, int guard) {
-    const TileCoord c = tile_coord(g);
+// Fused update of volume tile t (R rows per wave, NW waves per work-group).
+template <int POT, typename TI, typename TP, int R, int NW>
+__device__ __forceinline__ void update_tile(const DevGeom& g, const SlotTab& S, int n_slots,
+                                            const TP* __restrict__ hsub,
+                                            const TI* __restrict__ x_in, TI* __restrict__ x_out,
+                                            const float* __restrict__ d, const PriorClass& W,
+                                            float inv_delta, float beta, float capl_scale,
+                                            int guard, int t) {
+    const TileCoord c = tile_coord<R, NW>(g, t);
     if (c.iy0 >= g.ny) return;  // see sbp_kernel
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
     const VolReader<TP> P(hsub, (uint32_t)n_slots * g.nbx * g.nby);
-    const VolReader<TI> X(x_in, (uint32_t)g.nx * g.ny * g.nz);
-    float total[ROWS];
-    sbp_rows<TP>(g, S, n_slots, P, c.ix, c.iy0, c.iz, total);
+    const VolReader<TI> X(x_in, nvox);
+    const VolWriter<TI> XO(x_out, nvox);
+    const VolReader<float> D(d, nvox);
+    float total[R];
+    sbp_rows<TP, R>(g, S, n_slots, P, c.ix, c.iy0, c.iz, total);
     // wave-uniform: does the 3x3x3 neighbourhood of the wave's rows stay inside?
     const int x_first = c.ix - (threadIdx.x & 63);
     const bool interior = x_first >= 1 && x_first + 64 < g.nx && c.iy0 >= 1 &&
-                          c.iy0 + ROWS < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
+                          c.iy0 + R < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
     if (interior)
-        update_rows<POT, false, TI>(g, X, x_out, d, c, total, W, inv_delta, beta, capl_scale,
-                                    guard);
+        update_rows<POT, false, TI, R>(g, X, XO, D, c, total, W, inv_delta, beta, capl_scale,
+                                       guard);
     else
-        update_rows<POT, true, TI>(g, X, x_out, d, c, total, W, inv_delta, beta, capl_scale,
-                                   guard);
+        update_rows<POT, true, TI, R>(g, X, XO, D, c, total, W, inv_delta, beta, capl_scale,
+                                      guard);
+}
+
+template <int POT, typename TI, typename TP>
+__global__ __launch_bounds__(256) void update_fused_kernel(
+    DevGeom g, SlotTab S, int n_slots, const TP* __restrict__ hsub, const TI* __restrict__ x_in,
+    TI* __restrict__ x_out, const float* __restrict__ d, PriorClass W, float inv_delta,
+    float beta, float capl_scale, int guard) {
+    update_tile<POT, TI, TP, CASE_ROWS, CASE_WAVES>(g, S, n_slots, hsub, x_in, x_out, d, W,
+                                                    inv_delta, beta, capl_scale, guard,
+                                                    xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Batched fused update: every case of the table in one launch. The grid is
+// either one work-group per tile or persistent (a fixed number of groups per
+// CU, each looping over its share of the tiles, see BatchWalk); tiles of case
+// c are [cases[c].upd_begin, cases[c + 1].upd_begin). Scalars of the case are
+// taken into scalar registers once per tile.
+template <int POT, typename TI, typename TP, int R, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(5))) void batch_update_kernel(
+    const BatchCase* __restrict__ cases, const int* __restrict__ sched, int n_cases) {
+    BatchWalk walk(sched, n_cases);
+    int t, c;
+    while (walk.next(t, c)) {
+        const BatchCase& bc = cases[c];
+        const DevGeom g = uniform_copy(bc.g);
+        const PriorClass W = uniform_copy(bc.W);
+        const int n_slots = __builtin_amdgcn_readfirstlane(bc.n_slots);
+        const int guard = __builtin_amdgcn_readfirstlane(bc.guard);
+        const float inv_delta = __int_as_float(
+            __builtin_amdgcn_readfirstlane(__float_as_int(bc.inv_delta)));
+        const float beta = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(bc.beta)));
+        const float capl_scale = __int_as_float(
+            __builtin_amdgcn_readfirstlane(__float_as_int(bc.capl_scale)));
+        update_tile<POT, TI, TP, R, NW>(
+            g, bc.S, n_slots, (const TP*)uniform_ptr(bc.h), (const TI*)uniform_ptr(bc.x_in),
+            (TI*)uniform_ptr(bc.x_out), uniform_ptr(bc.d), W, inv_delta, beta, capl_scale,
+            guard, t);
+    }
 }
 
 // [slot][nbx][nby] -> [slot][nby][nbx], 64x64 tiles through LDS
@@ -334,8 +383,8 @@ bool fill_slots(SlotTab& S, const ViewTab* host_views, int a0, int a1, float src
     return true;
 }
 
-unsigned volume_tiles(const DevGeom& g) {
-    const int64_t tx = (g.nx + TILE_X - 1) / TILE_X, ty = (g.ny + TILE_Y - 1) / TILE_Y;
+unsigned volume_tiles(const DevGeom& g, int tile_y = CASE_ROWS * CASE_WAVES) {
+    const int64_t tx = (g.nx + TILE_X - 1) / TILE_X, ty = (g.ny + tile_y - 1) / tile_y;
     return (unsigned)(tx * ty * g.nz);
 }
 
@@ -349,7 +398,7 @@ hipError_t sbp_t(const DevGeom& g, const SlotTab& S, int n, float* img, const vo
 
 template <int POT, typename TI, typename TP>
 hipError_t fused_t(const DevGeom& g, const SlotTab& S, int n, const void* x_in, void* x_out,
-                   const float* d, const void* hsub, const PriorW& W, float inv_delta,
+                   const float* d, const void* hsub, const PriorClass& W, float inv_delta,
                    float beta, float capl_scale, int guard, hipStream_t st) {
     hipLaunchKernelGGL((update_fused_kernel<POT, TI, TP>), dim3(volume_tiles(g)), dim3(256), 0,
                        st, g, S, n, (const TP*)hsub, (const TI*)x_in, (TI*)x_out, d, W,
@@ -359,7 +408,7 @@ hipError_t fused_t(const DevGeom& g, const SlotTab& S, int n, const void* x_in, 
 
 template <int POT, typename TI>
 hipError_t fused_p(int prj_dtype, const DevGeom& g, const SlotTab& S, int n, const void* x_in,
-                   void* x_out, const float* d, const void* hsub, const PriorW& W,
+                   void* x_out, const float* d, const void* hsub, const PriorClass& W,
                    float inv_delta, float beta, float capl_scale, int guard, hipStream_t st) {
     switch (prj_dtype) {
         case OSTR_F32:
@@ -375,8 +424,8 @@ hipError_t fused_p(int prj_dtype, const DevGeom& g, const SlotTab& S, int n, con
 template <int POT>
 hipError_t fused_i(int img_dtype, int prj_dtype, const DevGeom& g, const SlotTab& S, int n,
                    const void* x_in, void* x_out, const float* d, const void* hsub,
-                   const PriorW& W, float inv_delta, float beta, float capl_scale, int guard,
-                   hipStream_t st) {
+                   const PriorClass& W, float inv_delta, float beta, float capl_scale,
+                   int guard, hipStream_t st) {
     switch (img_dtype) {
         case OSTR_F32:
             return fused_p<POT, float>(prj_dtype, g, S, n, x_in, x_out, d, hsub, W, inv_delta, beta, capl_scale, guard, st);
@@ -388,7 +437,64 @@ hipError_t fused_i(int img_dtype, int prj_dtype, const DevGeom& g, const SlotTab
     return hipErrorInvalidValue;
 }
 
+template <int POT, typename TP, int R, int NW>
+hipError_t batch_upd_t(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
+                       hipStream_t st) {
+    hipLaunchKernelGGL((batch_update_kernel<POT, float, TP, R, NW>), dim3(grid), dim3(64 * NW),
+                       0, st, cases, sched, n_cases);
+    return hipGetLastError();
+}
+
+template <int POT, typename TP>
+hipError_t batch_upd_s(int rows, int waves, const BatchCase* cases, const int* sched, int n_cases, int total,
+                       int grid, hipStream_t st) {
+    if (rows == 8 && waves == 4) return batch_upd_t<POT, TP, 8, 4>(cases, sched, n_cases, total, grid, st);
+    if (rows == 8 && waves == 2) return batch_upd_t<POT, TP, 8, 2>(cases, sched, n_cases, total, grid, st);
+    if (rows == 4 && waves == 4) return batch_upd_t<POT, TP, 4, 4>(cases, sched, n_cases, total, grid, st);
+    return hipErrorInvalidValue;
+}
+
+template <int POT>
+hipError_t batch_upd_p(int prj_dtype, int rows, int waves, const BatchCase* cases,
+                       const int* sched, int n_cases, int total, int grid, hipStream_t st) {
+    switch (prj_dtype) {
+        case OSTR_F32: return batch_upd_s<POT, float>(rows, waves, cases, sched, n_cases, total, grid, st);
+        case OSTR_BF16: return batch_upd_s<POT, bf16_t>(rows, waves, cases, sched, n_cases, total, grid, st);
+        case OSTR_F16: return batch_upd_s<POT, _Float16>(rows, waves, cases, sched, n_cases, total, grid, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 }  // namespace
+
+bool batch_update_shape_ok(int rows, int waves) {
+    return (rows == 8 && (waves == 4 || waves == 2)) || (rows == 4 && waves == 4);
+}
+
+int64_t batch_upd_tiles(const DevGeom& g, int rows, int waves) {
+    return volume_tiles(g, rows * waves);
+}
+
+void batch_slots(SlotTab& S, ViewTab* v, const ViewTab* host_views, int a0, int a1,
+                 float src_y) {
+    fill_slots(S, host_views, a0, a1, src_y);
+    for (int a = a0; a < a1; ++a) v[a - a0] = host_views[a];
+}
+
+hipError_t launch_batch_update(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
+                               int rows, int waves, int potential, int prj_dtype,
+                               hipStream_t st) {
+    if (total == 0) return hipSuccess;
+    switch (potential) {
+        case OSTR_PRIOR_Q2:
+            return batch_upd_p<OSTR_PRIOR_Q2>(prj_dtype, rows, waves, cases, sched, n_cases, total, grid, st);
+        case OSTR_PRIOR_LANGE:
+            return batch_upd_p<OSTR_PRIOR_LANGE>(prj_dtype, rows, waves, cases, sched, n_cases, total, grid, st);
+        case OSTR_PRIOR_HYPERBOLA:
+            return batch_upd_p<OSTR_PRIOR_HYPERBOLA>(prj_dtype, rows, waves, cases, sched, n_cases, total, grid, st);
+    }
+    return hipErrorInvalidValue;
+}
 
 size_t elem_size(int dtype) { return dtype == OSTR_F32 ? 4 : 2; }
 
@@ -440,8 +546,9 @@ hipError_t launch_update_fused(const DevGeom& g, const ViewTab* views, int a0, i
                                hipStream_t st) {
     SlotTab S;
     if (!fill_slots(S, views, a0, a1, g.src_y)) return hipErrorInvalidValue;
-    PriorW W;
-    prior_weights(W.w, rho);
+    PriorW W27;
+    prior_weights(W27.w, rho);
+    const PriorClass W = prior_classes(W27);
     float inv_delta = potential == OSTR_PRIOR_Q2 ? 0.0f : 1.0f / delta;
     const int n = a1 - a0;
     switch (potential) {

@@ -646,3 +646,244 @@ OSTR_API int ostr_diff_norms(ostr_handle* h, const void* a, const void* b, int32
     out[1] = host[1];
     return OSTR_OK;
 }
+
+// ---- batched cases ---------------------------------------------------------
+
+// A launch table: the tile schedule (2 * 8 * (n + 1) ints, see BatchWalk) at
+// offset 0 and the case entries at `case_off`.
+struct ostr_batch {
+    static constexpr int RING = 8;  // tables in flight
+    int device = 0;
+    int max_cases = 0;
+    int n_cu = 0;
+    size_t case_off = 0, slot_bytes = 0;
+    unsigned char* host[RING] = {};  // pinned staging
+    unsigned char* dev = nullptr;    // RING tables of slot_bytes
+    hipEvent_t ev[RING] = {};
+    bool used[RING] = {};
+    int pos = 0;
+    std::string err;
+};
+
+namespace {
+
+int bfail(ostr_batch* bt, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (bt)
+        bt->err = buf;
+    else
+        g_err = buf;
+    return code;
+}
+
+int bhip(ostr_batch* bt, hipError_t e, const char* where) {
+    return bfail(bt, OSTR_E_HIP, "%s: HIP error %d (%s)", where, (int)e, hipGetErrorString(e));
+}
+
+void batch_release(ostr_batch* bt) {
+    for (int k = 0; k < ostr_batch::RING; ++k) {
+        if (bt->ev[k]) (void)hipEventDestroy(bt->ev[k]);
+        if (bt->host[k]) (void)hipHostFree(bt->host[k]);
+    }
+    if (bt->dev) (void)hipFree(bt->dev);
+    delete bt;
+}
+
+// Validates the cases, fills the next staging table and copies it to the
+// device on `st`. kind 0 = forward, 1 = update. On success *table is the
+// device table, *total the number of tiles and *slot the ring position.
+int batch_grid(const ostr_batch* bt, int total, int groups_per_cu) {
+    if (groups_per_cu == 0) return total;
+    const int64_t g = (int64_t)bt->n_cu * groups_per_cu;
+    return (int)(g < total ? g : total);
+}
+
+int batch_stage(ostr_batch* bt, const ostr_batch_case* cases, int n, int prj_dtype,
+                int potential, const ostr_tile_cfg* cfg, int kind, hipStream_t st,
+                const char* fn, ostr::BatchCase** table, const int** sched, int* total,
+                int* grid_out, int* slot) {
+    if (!bt) return bfail(nullptr, OSTR_E_ARG, "%s: NULL batch", fn);
+    if (!cases || !cfg) return bfail(bt, OSTR_E_ARG, "%s: NULL argument", fn);
+    if (n < 1 || n > bt->max_cases)
+        return bfail(bt, OSTR_E_RANGE, "%s: %d cases, batch holds 1..%d", fn, n, bt->max_cases);
+    if (prj_dtype != OSTR_F32 && prj_dtype != OSTR_BF16 && prj_dtype != OSTR_F16)
+        return bfail(bt, OSTR_E_UNSUPPORTED, "%s: unknown dtype %d", fn, prj_dtype);
+    if (kind == 0) {
+        const int r = cfg->fwd_rows;
+        if (r != 1 && r != 2 && r != 4 && r != 8)
+            return bfail(bt, OSTR_E_ARG, "%s: fwd_rows %d not in {1,2,4,8}", fn, r);
+        if (cfg->fwd_groups_per_cu < 0 || cfg->fwd_groups_per_cu > 64)
+            return bfail(bt, OSTR_E_ARG, "%s: fwd_groups_per_cu outside 0..64", fn);
+    } else {
+        if (!ostr::batch_update_shape_ok(cfg->upd_rows, cfg->upd_waves))
+            return bfail(bt, OSTR_E_ARG, "%s: update shape %dx%d not in {8x4, 8x2, 4x4}", fn,
+                         cfg->upd_rows, cfg->upd_waves);
+        if (cfg->upd_groups_per_cu < 0 || cfg->upd_groups_per_cu > 64)
+            return bfail(bt, OSTR_E_ARG, "%s: upd_groups_per_cu outside 0..64", fn);
+        if (potential < OSTR_PRIOR_Q2 || potential > OSTR_PRIOR_HYPERBOLA)
+            return bfail(bt, OSTR_E_ARG, "%s: unknown potential %d", fn, potential);
+    }
+    for (int c = 0; c < n; ++c) {
+        const ostr_batch_case& k = cases[c];
+        if (!k.h) return bfail(bt, OSTR_E_ARG, "%s: case %d: NULL handle", fn, c);
+        if (k.h->device != bt->device)
+            return bfail(bt, OSTR_E_ARG, "%s: case %d is on device %d, batch on %d", fn, c,
+                         k.h->device, bt->device);
+        if (k.a0 < 0 || k.a1 > k.h->n_views || k.a0 > k.a1 || k.a1 - k.a0 > ostr::MAX_SLOTS)
+            return bfail(bt, OSTR_E_RANGE, "%s: case %d: slots [%d,%d) invalid (n_views %d, "
+                         "at most %d per subset)", fn, c, k.a0, k.a1, k.h->n_views,
+                         ostr::MAX_SLOTS);
+        if (!k.x_in || !k.hsub || (kind == 0 && (!k.y || !k.b)) ||
+            (kind == 1 && (!k.x_out || !k.d)))
+            return bfail(bt, OSTR_E_ARG, "%s: case %d: NULL buffer", fn, c);
+        if (kind == 1) {
+            if (k.x_in == k.x_out)
+                return bfail(bt, OSTR_E_ARG, "%s: case %d: x_in and x_out alias", fn, c);
+            if (potential != OSTR_PRIOR_Q2 && k.delta == 0.0f)
+                return bfail(bt, OSTR_E_ARG, "%s: case %d: delta cannot be zero", fn, c);
+            if (!(k.z_xy_ratio > 0.0f))
+                return bfail(bt, OSTR_E_ARG, "%s: case %d: z_xy_ratio must be > 0", fn, c);
+        }
+    }
+    hipError_t e = hipSetDevice(bt->device);
+    if (e != hipSuccess) return bhip(bt, e, fn);
+    const int k = bt->pos;
+    if (bt->used[k]) {  // the table of RING calls ago must be consumed
+        e = hipEventSynchronize(bt->ev[k]);
+        if (e != hipSuccess) return bhip(bt, e, fn);
+    }
+    ostr::BatchCase* H = (ostr::BatchCase*)(bt->host[k] + bt->case_off);
+    std::vector<int64_t> cnt(n);
+    int64_t tiles = 0;
+    for (int c = 0; c < n; ++c) {
+        const ostr_batch_case& in = cases[c];
+        ostr::BatchCase& o = H[c];
+        const ostr::DevGeom& g = in.h->dg;
+        o.g = g;
+        o.n_slots = in.a1 - in.a0;
+        cnt[c] = kind == 0 ? ostr::batch_fwd_tiles(g, o.n_slots, cfg->fwd_rows)
+                           : ostr::batch_upd_tiles(g, cfg->upd_rows, cfg->upd_waves);
+        tiles += cnt[c];
+        if (tiles > 0x7fffffff) return bfail(bt, OSTR_E_RANGE, "%s: too many tiles", fn);
+        o.guard = in.guard;
+        o.beta = in.beta;
+        o.capl_scale = in.capl_scale;
+        o.inv_delta = (kind == 1 && potential != OSTR_PRIOR_Q2) ? 1.0f / in.delta : 0.0f;
+        o.x_in = in.x_in;
+        o.x_out = in.x_out;
+        o.d = in.d;
+        o.y = in.y;
+        o.b = in.b;
+        o.r = in.r;
+        o.h = in.hsub;
+        ostr::batch_slots(o.S, o.v, in.h->views_host.data(), in.a0, in.a1, g.src_y);
+        if (kind == 1) {
+            ostr::PriorW w27;
+            ostr::prior_weights(w27.w, in.z_xy_ratio);
+            o.W = ostr::prior_classes(w27);
+        }
+    }
+    const int gpc = kind == 0 ? cfg->fwd_groups_per_cu : cfg->upd_groups_per_cu;
+    const int grid = batch_grid(bt, (int)tiles, gpc);
+    ostr::batch_schedule((int*)bt->host[k], cnt.data(), n, grid);
+    unsigned char* D = bt->dev + (size_t)k * bt->slot_bytes;
+    e = hipMemcpyAsync(D, bt->host[k], bt->case_off + sizeof(ostr::BatchCase) * n,
+                       hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return bhip(bt, e, fn);
+    *table = (ostr::BatchCase*)(D + bt->case_off);
+    *sched = (const int*)D;
+    *total = (int)tiles;
+    *grid_out = grid;
+    *slot = k;
+    return OSTR_OK;
+}
+
+int batch_finish(ostr_batch* bt, int slot, hipError_t launch, hipStream_t st, const char* fn) {
+    if (launch != hipSuccess) return bhip(bt, launch, fn);
+    hipError_t e = hipEventRecord(bt->ev[slot], st);
+    if (e != hipSuccess) return bhip(bt, e, fn);
+    bt->used[slot] = true;
+    bt->pos = (slot + 1) % ostr_batch::RING;
+    return OSTR_OK;
+}
+
+}  // namespace
+
+OSTR_API int ostr_batch_create(int32_t device, int32_t max_cases, ostr_batch** out) {
+    if (!out) return bfail(nullptr, OSTR_E_ARG, "ostr_batch_create: NULL output");
+    *out = nullptr;
+    if (max_cases < 1 || max_cases > 4096)
+        return bfail(nullptr, OSTR_E_ARG, "ostr_batch_create: max_cases outside 1..4096");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess) return bhip(nullptr, e, "ostr_batch_create");
+    if (device < 0 || device >= ndev)
+        return bfail(nullptr, OSTR_E_ARG, "ostr_batch_create: no device %d", device);
+    ostr_batch* bt = new ostr_batch;
+    bt->device = device;
+    bt->max_cases = max_cases;
+    const size_t sched_bytes = sizeof(int) * 16 * ((size_t)max_cases + 1);
+    bt->case_off = (sched_bytes + 255) & ~(size_t)255;
+    bt->slot_bytes = bt->case_off + sizeof(ostr::BatchCase) * (size_t)max_cases;
+    bt->slot_bytes = (bt->slot_bytes + 255) & ~(size_t)255;
+    e = hipSetDevice(device);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&bt->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipMalloc((void**)&bt->dev, bt->slot_bytes * ostr_batch::RING);
+    for (int k = 0; k < ostr_batch::RING && e == hipSuccess; ++k) {
+        e = hipHostMalloc((void**)&bt->host[k], bt->slot_bytes, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&bt->ev[k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        bhip(nullptr, e, "ostr_batch_create");
+        batch_release(bt);
+        return OSTR_E_HIP;
+    }
+    *out = bt;
+    return OSTR_OK;
+}
+
+OSTR_API void ostr_batch_destroy(ostr_batch* bt) {
+    if (!bt) return;
+    (void)hipSetDevice(bt->device);
+    for (int k = 0; k < ostr_batch::RING; ++k)
+        if (bt->used[k]) (void)hipEventSynchronize(bt->ev[k]);
+    batch_release(bt);
+}
+
+OSTR_API const char* ostr_batch_last_error(const ostr_batch* bt) {
+    return bt ? bt->err.c_str() : g_err.c_str();
+}
+
+OSTR_API int ostr_batch_fproj_h(ostr_batch* bt, const ostr_batch_case* cases, int32_t n_cases,
+                                int32_t prj_dtype, const ostr_tile_cfg* cfg, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    ostr::BatchCase* table;
+    const int* sched;
+    int total, grid, slot, rc;
+    if ((rc = batch_stage(bt, cases, n_cases, prj_dtype, 0, cfg, 0, st, "ostr_batch_fproj_h",
+                          &table, &sched, &total, &grid, &slot)))
+        return rc;
+    hipError_t e = ostr::launch_batch_project(table, sched, n_cases, total, grid,
+                                              cfg->fwd_rows, prj_dtype, st);
+    return batch_finish(bt, slot, e, st, "ostr_batch_fproj_h");
+}
+
+OSTR_API int ostr_batch_update_fused(ostr_batch* bt, const ostr_batch_case* cases,
+                                     int32_t n_cases, int32_t prj_dtype, int32_t potential,
+                                     const ostr_tile_cfg* cfg, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    ostr::BatchCase* table;
+    const int* sched;
+    int total, grid, slot, rc;
+    if ((rc = batch_stage(bt, cases, n_cases, prj_dtype, potential, cfg, 1, st,
+                          "ostr_batch_update_fused", &table, &sched, &total, &grid, &slot)))
+        return rc;
+    hipError_t e = ostr::launch_batch_update(table, sched, n_cases, total, grid, cfg->upd_rows,
+                                             cfg->upd_waves, potential, prj_dtype, st);
+    return batch_finish(bt, slot, e, st, "ostr_batch_update_fused");
+}

@@ -24,8 +24,6 @@ namespace ostr {
 
 namespace {
 
-constexpr int MAX_SLOTS = 64;
-
 struct MatchedSlots {
     float x0[MAX_SLOTS], z0[MAX_SLOTS], c[MAX_SLOTS];
 };

@@ -25,6 +25,79 @@ struct ViewTab {
     float x0, z0, c, s;
 };
 
+constexpr int MAX_SLOTS = 64;  // views per backprojection pass
+
+// Slot parameters of the backprojector for the slots of one launch.
+struct SlotTab {
+    float x0[MAX_SLOTS], z0[MAX_SLOTS];
+    float num[MAX_SLOTS];  // |S|^2 evaluated as fma(z0,z0,fma(y0,y0,x0*x0))
+};
+
+// 27-neighbour prior weights, index dz*9 + dy*3 + dx (centre 13 is 0).
+struct PriorW {
+    float w[27];
+};
+
+// The same weights by class: with dx == dy a weight depends only on whether
+// the neighbour is in the voxel's slice and on how many of its x, y offsets
+// are non-zero, and prior_weights computes every member of a class from the
+// same operands, so the 5 class values are the 27 weights bit for bit.
+// Class of index dz*9+dy*3+dx: in-slice (dz = 1) m-1, else 2+m, where m is the
+// number of non-zero x, y offsets.
+struct PriorClass {
+    float w[5];
+};
+__host__ __device__ constexpr int prior_class(int nidx) {
+    return (nidx / 9 == 1) ? ((nidx % 3 != 1) + ((nidx / 3) % 3 != 1)) - 1
+                           : 2 + ((nidx % 3 != 1) + ((nidx / 3) % 3 != 1));
+}
+inline PriorClass prior_classes(const PriorW& W) {
+    // representatives: in-slice edge 10, in-slice diagonal 9; other slice
+    // centre 4, edge 1, corner 0
+    return PriorClass{{W.w[10], W.w[9], W.w[4], W.w[1], W.w[0]}};
+}
+
+// One case of a batched launch (device-resident table, rebuilt per call).
+struct BatchCase {
+    DevGeom g;
+    int n_slots;
+    int guard;
+    float beta, capl_scale, inv_delta;
+    int pad_;
+    const void* x_in;
+    void* x_out;
+    const float* d;
+    const float* y;
+    const float* b;
+    const float* r;
+    void* h;
+    PriorClass W;
+    ViewTab v[MAX_SLOTS];  // slot table for the forward walk
+    SlotTab S;             // slot table for the backprojector
+};
+
+// Wave-uniform copy of a struct of 4-byte words from memory: each word is
+// loaded and taken with readfirstlane, so the copy lives in scalar registers
+// (as kernel arguments do) and the compiler knows it is uniform.
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T& src) {
+    static_assert(sizeof(T) % 4 == 0, "struct of 4-byte words");
+    T out;
+    const int* s = reinterpret_cast<const int*>(&src);
+    int* d = reinterpret_cast<int*>(&out);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) d[k] = __builtin_amdgcn_readfirstlane(s[k]);
+    return out;
+}
+
+template <typename P>
+__device__ __forceinline__ P* uniform_ptr(P* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (P*)(((uint64_t)hi << 32) | lo);
+}
+
 DevGeom make_dev_geom(const ostr_geom& g);
 
 // ---- storage type helpers ------------------------------------------------
@@ -53,15 +126,15 @@ template <>
 __device__ __forceinline__ void st<float>(float* p, int64_t i, float v) {
     p[i] = v;
 }
+// float -> bf16 bits, round to nearest even, NaN -> quiet NaN
+__device__ __forceinline__ uint16_t bf16_bits(float v) {
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
 template <>
 __device__ __forceinline__ void st<bf16_t>(bf16_t* p, int64_t i, float v) {
-    uint32_t u = __float_as_uint(v);
-    uint16_t b;
-    if ((u & 0x7fffffffu) > 0x7f800000u)
-        b = 0x7fc0;  // quiet NaN
-    else
-        b = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // round to nearest even
-    p[i].bits = b;
+    p[i].bits = bf16_bits(v);
 }
 template <>
 __device__ __forceinline__ void st<_Float16>(_Float16* p, int64_t i, float v) {
@@ -94,6 +167,31 @@ template <>
 __device__ __forceinline__ float VolReader<_Float16>::operator()(uint32_t i) const {
     unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(rsrc, i * 2u, 0, 0);
     return (float)__builtin_bit_cast(_Float16, b);
+}
+
+// Volume stores through a raw buffer resource (vector buffer stores, 32-bit
+// byte offsets), converting from float like st<T>.
+template <typename T>
+struct VolWriter {
+    __amdgpu_buffer_rsrc_t rsrc;
+    __device__ VolWriter(T* p, uint32_t n) {
+        rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * sizeof(T)),
+                                                 0x00020000);
+    }
+    __device__ __forceinline__ void operator()(uint32_t i, float v) const;
+};
+template <>
+__device__ __forceinline__ void VolWriter<float>::operator()(uint32_t i, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rsrc, i * 4u, 0, 0);
+}
+template <>
+__device__ __forceinline__ void VolWriter<bf16_t>::operator()(uint32_t i, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b16(bf16_bits(v), rsrc, i * 2u, 0, 0);
+}
+template <>
+__device__ __forceinline__ void VolWriter<_Float16>::operator()(uint32_t i, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)v), rsrc,
+                                          i * 2u, 0, 0);
 }
 
 // Sum of one double per thread over a 256-thread work-group in a fixed order
@@ -137,6 +235,24 @@ hipError_t launch_update_fused(const DevGeom& g, const ViewTab* views, int a0, i
                                const void* x_in, void* x_out, int img_dtype, const float* d,
                                const void* hsub, int prj_dtype, float beta, float capl_scale,
                                int potential, float delta, float rho, int guard,
+                               hipStream_t st);
+
+// Batched launches over a device table of `n_cases` BatchCase entries with
+// `total` tiles; `grid` work-groups (one per tile, or a persistent grid).
+// Forward: projection + h (kind 3), FP32 image, `rows` detector rows per group
+// (1, 2, 4, 8). Update: FP32 image, `rows` x `waves` shape (see
+// batch_update_shape_ok).
+int64_t batch_fwd_tiles(const DevGeom& g, int n_slots, int rows);
+int64_t batch_upd_tiles(const DevGeom& g, int rows, int waves);
+bool batch_update_shape_ok(int rows, int waves);
+// Fills the backprojector and walker slot tables for slots [a0, a1) of a
+// handle's host view table (a1 - a0 <= MAX_SLOTS).
+void batch_slots(SlotTab& S, ViewTab* v, const ViewTab* host_views, int a0, int a1,
+                 float src_y);
+hipError_t launch_batch_project(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
+                                int rows, int prj_dtype, hipStream_t st);
+hipError_t launch_batch_update(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
+                               int rows, int waves, int potential, int prj_dtype,
                                hipStream_t st);
 
 // Prior.
@@ -198,6 +314,77 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     int full = nblocks - (per - 1) * nxcd;  // XCDs owning `per` tiles
     int t = xcd < full ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
     return t;
+}
+
+// Tile walk of a batched launch (one-tile-per-group and persistent grids
+// alike). Work-group b runs on XCD b % 8. The tiles of each case are cut into
+// nxcd = min(8, grid) contiguous chunks and XCD x owns chunk x of every case;
+// the groups of an XCD walk its chunks in case order, interleaved (group k of
+// n takes local tiles k, k+n, k+2n, ...). Tiles of one case cost about the
+// same and tiles of different cases do not (view count, thickness), so every
+// XCD gets the same mix of work, and the groups in flight on one XCD work on
+// neighbouring tiles of one case (L2 reuse).
+//
+// The host writes the schedule (batch_schedule): for XCD x, xoff[x][c] is
+// the position of case c's chunk in the XCD's list (xoff[x][n_cases] = list
+// length) and xlo[x][c] the first tile of that chunk in the case's own tile
+// numbering; rows are n_cases + 1 ints, xoff rows first (8 rows), then xlo.
+// next() finds the case by binary search, kept while the group stays in it,
+// and returns false once the group's share is done; every group reaches that.
+struct BatchWalk {
+    const int* xoff;
+    const int* xlo;
+    int n_cases, n, u, len, c, c_begin, c_end, c_lo;
+
+    __device__ int at(const int* p, int i) const { return __builtin_amdgcn_readfirstlane(p[i]); }
+    __device__ BatchWalk(const int* sched, int nc) : n_cases(nc) {
+        const int G = gridDim.x;
+        const int nxcd = G < 8 ? G : 8;
+        const int x = blockIdx.x % nxcd;
+        n = G / nxcd + (x < G % nxcd ? 1 : 0);
+        u = blockIdx.x / nxcd;
+        xoff = sched + x * (nc + 1);
+        xlo = sched + (8 + x) * (nc + 1);
+        len = at(xoff, nc);
+        c = 0;
+        c_begin = c_end = 0;
+    }
+    __device__ bool next(int& t, int& case_idx) {
+        if (u >= len) return false;
+        if (u >= c_end) {  // largest c with xoff[c] <= u (a non-empty chunk)
+            int lo = 0, hi = n_cases;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (at(xoff, mid) <= u) lo = mid; else hi = mid;
+            }
+            c = lo;
+            c_begin = at(xoff, c);
+            c_end = at(xoff, c + 1);
+            c_lo = at(xlo, c);
+        }
+        t = c_lo + (u - c_begin);
+        case_idx = c;
+        u += n;
+        return true;
+    }
+};
+
+// Host side of BatchWalk: fills sched (2 * 8 * (n_cases + 1) ints) for tile
+// counts `tiles` of the cases and a grid of `grid` work-groups.
+inline void batch_schedule(int* sched, const int64_t* tiles, int n_cases, int grid) {
+    const int nxcd = grid < 8 ? grid : 8;
+    const int row = n_cases + 1;
+    for (int x = 0; x < 8; ++x) {
+        int off = 0;
+        for (int c = 0; c < n_cases; ++c) {
+            const int lo = (int)(tiles[c] * x / nxcd), hi = (int)(tiles[c] * (x + 1) / nxcd);
+            sched[x * row + c] = off;
+            sched[(8 + x) * row + c] = lo;
+            off += x < nxcd ? hi - lo : 0;
+        }
+        sched[x * row + n_cases] = off;
+        sched[(8 + x) * row + n_cases] = 0;
+    }
 }
 
 }  // namespace ostr

@@ -13,10 +13,6 @@
 
 namespace ostr {
 
-struct PriorW {
-    float w[27];
-};
-
 template <int POT>
 __device__ __forceinline__ float prior_omega(float t, float inv_delta) {
     if (POT == OSTR_PRIOR_Q2) return 1.0f;

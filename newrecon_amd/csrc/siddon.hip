@@ -222,27 +222,13 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
 // detector-row-major layout [slot][nby][nbx]); 4 / 5 gamma (ray length / row
 // sum) times yry(y, r) with y, r read in the reference layout and the product
 // written row-major (the fused precompute).
+// Bin (i, j) of launch slot `slot` (view parameters v).
 template <int KIND, typename TI, typename TO>
-__global__ __launch_bounds__(256) void project_kernel(DevGeom g, const ViewTab* __restrict__ views,
-                                                      int a0, int n_slots, int tiles_i,
-                                                      int tiles_j, int cos_weight,
-                                                      const TI* __restrict__ img,
-                                                      TO* __restrict__ out,
-                                                      const float* __restrict__ y,
-                                                      const float* __restrict__ b,
-                                                      const float* __restrict__ r) {
-    const int nblocks = gridDim.x;
-    const int t = xcd_tile(blockIdx.x, nblocks);
-    // tile order: j-tile fastest, then view slot, then i-tile
-    const int tj = t % tiles_j;
-    const int rest = t / tiles_j;
-    const int slot = rest % n_slots;
-    const int ti = rest / n_slots;
-    const int i = ti * TILE_I + (threadIdx.x & 63);
-    const int j = tj * TILE_J + (threadIdx.x >> 6);
-    if (i >= g.nbx || j >= g.nby) return;
-
-    const ViewTab v = views[a0 + slot];
+__device__ __forceinline__ void project_bin(const DevGeom& g, const ViewTab& v, int slot, int i,
+                                            int j, int cos_weight, const TI* __restrict__ img,
+                                            TO* __restrict__ out, const float* __restrict__ y,
+                                            const float* __restrict__ b,
+                                            const float* __restrict__ r) {
     const float ex = bin_centre(g.x_d0, i, g.dbx);
     const float ey = bin_centre(g.y_d0, j, g.dby);
     const float ez = -g.det_r;
@@ -286,6 +272,57 @@ __global__ __launch_bounds__(256) void project_kernel(DevGeom g, const ViewTab* 
         val = c * val;
     }
     st<TO>(out, (KIND >= 3) ? o_row : o_ref, val);
+}
+
+// Tile t of a launch over n_slots views, TJ detector rows x 64 columns per
+// work-group; tile order: j-tile fastest, then view slot, then i-tile (the
+// groups working at one time share a view and neighbouring columns).
+template <int TJ>
+__device__ __forceinline__ bool bin_of_tile(const DevGeom& g, int n_slots, int t, int& slot,
+                                            int& i, int& j) {
+    const int tiles_j = (g.nby + TJ - 1) / TJ;
+    const int tj = t % tiles_j;
+    const int rest = t / tiles_j;
+    slot = rest % n_slots;
+    const int ti = rest / n_slots;
+    i = ti * TILE_I + (threadIdx.x & 63);
+    j = tj * TJ + (threadIdx.x >> 6);
+    return i < g.nbx && j < g.nby;
+}
+
+template <int KIND, typename TI, typename TO>
+__global__ __launch_bounds__(256) void project_kernel(DevGeom g, const ViewTab* __restrict__ views,
+                                                      int a0, int n_slots, int tiles_i,
+                                                      int tiles_j, int cos_weight,
+                                                      const TI* __restrict__ img,
+                                                      TO* __restrict__ out,
+                                                      const float* __restrict__ y,
+                                                      const float* __restrict__ b,
+                                                      const float* __restrict__ r) {
+    int slot, i, j;
+    if (!bin_of_tile<TILE_J>(g, n_slots, xcd_tile(blockIdx.x, gridDim.x), slot, i, j)) return;
+    project_bin<KIND, TI, TO>(g, views[a0 + slot], slot, i, j, cos_weight, img, out, y, b, r);
+}
+
+// Batched projection + h (kind 3) over every case of the table in one launch;
+// schedule as batch_update_kernel.
+template <typename TI, typename TO, int TJ>
+__global__ __launch_bounds__(64 * TJ) void batch_project_kernel(
+    const BatchCase* __restrict__ cases, const int* __restrict__ sched, int n_cases) {
+    BatchWalk walk(sched, n_cases);
+    int t, c;
+    while (walk.next(t, c)) {
+        const BatchCase& bc = cases[c];
+        const DevGeom g = uniform_copy(bc.g);
+        const int n_slots = __builtin_amdgcn_readfirstlane(bc.n_slots);
+        int slot, i, j;
+        const bool in = bin_of_tile<TJ>(g, n_slots, t, slot, i, j);
+        const ViewTab v = uniform_copy(bc.v[slot]);  // slot is uniform over the group
+        if (!in) continue;
+        project_bin<3, TI, TO>(g, v, slot, i, j, 1, (const TI*)uniform_ptr(bc.x_in),
+                               (TO*)uniform_ptr(bc.h), uniform_ptr(bc.y), uniform_ptr(bc.b),
+                               uniform_ptr(bc.r));
+    }
 }
 
 __global__ __launch_bounds__(256) void count_kernel(DevGeom g, const ViewTab* __restrict__ views,
@@ -380,6 +417,43 @@ hipError_t launch_project(const DevGeom& g, const ViewTab* views, int a0, int a1
             return launch_kind<4, float, float>(g, views, a0, a1, 0, nullptr, out, y, b, r, st);
         case 5:
             return launch_kind<5, float, float>(g, views, a0, a1, 0, nullptr, out, y, b, r, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+namespace {
+template <typename TO, int TJ>
+hipError_t batch_fwd_t(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
+                       hipStream_t st) {
+    hipLaunchKernelGGL((batch_project_kernel<float, TO, TJ>), dim3(grid), dim3(64 * TJ), 0, st,
+                       cases, sched, n_cases);
+    return hipGetLastError();
+}
+
+template <typename TO>
+hipError_t batch_fwd_o(int rows, const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
+                       hipStream_t st) {
+    switch (rows) {
+        case 1: return batch_fwd_t<TO, 1>(cases, sched, n_cases, total, grid, st);
+        case 2: return batch_fwd_t<TO, 2>(cases, sched, n_cases, total, grid, st);
+        case 4: return batch_fwd_t<TO, 4>(cases, sched, n_cases, total, grid, st);
+        case 8: return batch_fwd_t<TO, 8>(cases, sched, n_cases, total, grid, st);
+    }
+    return hipErrorInvalidValue;
+}
+}  // namespace
+
+int64_t batch_fwd_tiles(const DevGeom& g, int n_slots, int rows) {
+    return (int64_t)((g.nbx + TILE_I - 1) / TILE_I) * ((g.nby + rows - 1) / rows) * n_slots;
+}
+
+hipError_t launch_batch_project(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
+                                int rows, int prj_dtype, hipStream_t st) {
+    if (total == 0) return hipSuccess;
+    switch (prj_dtype) {
+        case OSTR_F32: return batch_fwd_o<float>(rows, cases, sched, n_cases, total, grid, st);
+        case OSTR_BF16: return batch_fwd_o<bf16_t>(rows, cases, sched, n_cases, total, grid, st);
+        case OSTR_F16: return batch_fwd_o<_Float16>(rows, cases, sched, n_cases, total, grid, st);
     }
     return hipErrorInvalidValue;
 }

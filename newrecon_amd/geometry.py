@@ -178,6 +178,30 @@ def scan_tiny(nx=12, ny=10, nz=6, nbx=24, nby=16, n_views=3) -> Scan:
                 (2, 3), 1, 7, 1007, OSTRConfig(n_subsets=n_views, n_iters=3))
 
 
+BATCH_THICKNESS = (30, 40, 45, 50, 55, 60, 70, 80)
+
+
+def scan_batch(scale: int = 1) -> List[Scan]:
+    """The uneven batch: 8 cases of 30..80 slices of 1 mm, in-plane sizes
+    linearly spaced from 800x400 to 1400x700 voxels of 0.1 mm, view counts
+    cycling 9/15/25 over +-7.5/15/25 deg with 3/5/5 subsets, 2048x1664
+    detector of 0.14 mm, I0 = 5e3; seeds 40..47 (phantom), 1040..1047 (noise).
+    `scale` > 1 divides the in-plane sizes, thickness and detector by it (voxel
+    and bin sizes times it), a reduced copy of the same table for tests."""
+    out = []
+    for k, nz in enumerate(BATCH_THICKNESS):
+        nx = int(round(800 + 600 * k / 7)) // scale
+        ny = int(round(400 + 300 * k / 7)) // scale
+        nv, span, m = ((9, 7.5, 3), (15, 15.0, 5), (25, 25.0, 5))[k % 3]
+        g = Geometry.half_cone(nx, ny, max(nz // scale, 2), 0.1 * scale, 0.1 * scale,
+                               1.0 * scale, 2048 // scale, 1664 // scale, 0.14 * scale,
+                               0.14 * scale)
+        out.append(Scan(f"dbt-batch{k}-{g.nx}x{g.ny}x{g.nz}-{nv}v", g, arc_angles(nv, span),
+                        5e3, (50, 200), 20, 40 + k, 1040 + k,
+                        OSTRConfig(n_subsets=m, n_iters=20)))
+    return out
+
+
 SCANS = {"small": scan_small, "clinical": scan_clinical, "tiny": scan_tiny}
 
 
