@@ -98,7 +98,7 @@ class BatchOSTR:
         self.cases: List[OSTR] = list(cases)
         c0 = self.cases[0]
         for c in self.cases:
-            if not c.cfg.fused:
+            if not c.fused:
                 raise ValueError("batched cases must use the fused kernels")
             if c.img_dtype != torch.float32:
                 raise ValueError("batched cases take float32 images")

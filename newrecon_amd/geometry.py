@@ -107,6 +107,7 @@ class Geometry:
 
 POTENTIALS = {"q2": 0, "lange": 1, "hyperbola": 2}
 GAMMA_MODES = {"ray_length": 0, "row_sum": 1}
+BACKPROJECTORS = {"sbp": 0, "matched": 1}
 
 
 @dataclasses.dataclass
@@ -121,6 +122,11 @@ class OSTRConfig:
     gamma_mode: str = "row_sum"   # "ray_length" reproduces Tomo_OSTR.cu:1172
     guard: bool = True            # False reproduces the 0/0 NaN of update_est
     fused: bool = True            # fused kernels vs. launcher-by-launcher path
+    # "sbp": the reference's bilinear backprojector for d and capL (not the
+    # transpose of the forward model, so the iteration is not a descent method
+    # and does not converge); "matched": the exact transpose G^T of
+    # G = cos(t) A, d = G^T(yry * G 1) (gamma_mode unused): OS-SQS proper.
+    backprojector: str = "sbp"
 
     @property
     def potential(self) -> int:

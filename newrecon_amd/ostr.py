@@ -29,7 +29,8 @@ from typing import Callable, List, Optional
 import numpy as np
 import torch
 
-from .geometry import GAMMA_MODES, POTENTIALS, Geometry, OSTRConfig, regroup_index
+from .geometry import (BACKPROJECTORS, GAMMA_MODES, POTENTIALS, Geometry, OSTRConfig,
+                       regroup_index)
 
 LIB_PATH = os.environ.get("OSTR_HIP_LIB") or os.path.join(
     os.path.dirname(os.path.abspath(__file__)), "libostr_hip.so")
@@ -369,8 +370,14 @@ class OSTR:
             raise ValueError("voxel size in x must equal voxel size in y (prior weights)")
         if self.cfg.prior != "q2" and self.cfg.delta == 0:
             raise ValueError("delta cannot be zero")
-        if not self.cfg.fused and (img_dtype != torch.float32 or prj_dtype != torch.float32):
-            raise ValueError("BF16/FP16 storage is implemented by the fused kernels only")
+        if self.cfg.backprojector not in BACKPROJECTORS:
+            raise ValueError(f"unknown backprojector {self.cfg.backprojector!r}")
+        self.matched = self.cfg.backprojector == "matched"
+        # the fused kernels implement the SBP update; the matched update runs
+        # launch by launch (forward, h, exact transpose, prior, update)
+        self.fused = self.cfg.fused and not self.matched
+        if not self.fused and (img_dtype != torch.float32 or prj_dtype != torch.float32):
+            raise ValueError("BF16/FP16 storage is implemented by the fused SBP kernels only")
         self.sys = DBTSystem(geom, angles, device=device)
         self.dev = self.sys.device
         self.M, self.vs = M, nv // M
@@ -416,8 +423,14 @@ class OSTR:
         """gamma, yry, d = SBP(gamma*yry) (natural order), then the OS regroup."""
         s, cfg = self.sys, self.cfg
         nv = self.sys.n_views
-        if cfg.fused:
+        if self.fused:
             self.d = s.precompute_d(self.y_nat, self.r_nat, cfg.gamma_mode)
+        elif self.matched:  # d = G^T(yry * G 1)
+            gam = s.fproj(s.volume(fill=1.0), 0, nv, cos_weight=True)
+            yry = s.compute_yry(torch.empty_like(self.y_nat), self.y_nat, self.r_nat)
+            s.compute_gamma_yry(gam, yry, gam)
+            self.d = s.bproj_matched(gam, 0, nv, cos_weight=True)
+            del gam, yry
         else:
             gam = s.gamma(0, nv, cfg.gamma_mode)
             yry = s.compute_yry(torch.empty_like(self.y_nat), self.y_nat, self.r_nat)
@@ -425,12 +438,12 @@ class OSTR:
             self.d = s.bprojectCB_GPU_SBP(s.volume(), gam, 0, nv)
             del gam, yry
         # the fused kernels read subset sinograms detector-row-major
-        rows = cfg.fused
+        rows = self.fused
         self.y = s.regroup_prj(self.y_nat, self.M, to_rows=rows)
         self.b = s.regroup_prj(self.b_nat, self.M, to_rows=rows)
         self.r = None if self.r_nat is None else s.regroup_prj(self.r_nat, self.M, to_rows=rows)
         s.set_index(regroup_index(nv, self.M))
-        if cfg.fused:
+        if self.fused:
             self.h = torch.zeros((self.vs, self.geom.nby, self.geom.nbx), dtype=self.prj_dtype,
                                  device=self.dev)
             self.x_next = torch.empty_like(self.x)
@@ -446,7 +459,7 @@ class OSTR:
         a0, a1 = m * self.vs, (m + 1) * self.vs
         sl = slice(a0, a1)
         r_sub = None if self.r is None else self.r[sl]
-        if cfg.fused:
+        if self.fused:
             self._timed("fproj_h", s.fproj_h, self.x, self.y[sl], self.b[sl], r_sub, a0, a1,
                         hsub=self.h)
             self._timed("update_fused", s.update_fused, self.x, self.x_next, self.d, self.h, a0,
@@ -456,7 +469,10 @@ class OSTR:
             s.fprojectCB_1R_GPU_OSTR_cos(self.x, self.line, a0, a1)
             s.compute_h(self.h, self.y[sl], self.b[sl], self.line, r_sub)
             self.capL.zero_()
-            s.bprojectCB_GPU_SBP(self.capL, self.h, a0, a1)
+            if self.matched:
+                s.bproj_matched(self.h, a0, a1, img=self.capL, cos_weight=True)
+            else:
+                s.bprojectCB_GPU_SBP(self.capL, self.h, a0, a1)
             self.RDD.zero_()
             self.RD.zero_()
             s.prior(self.x, cfg.prior, cfg.delta if cfg.prior != "q2" else 1.0, self.rho,

@@ -45,7 +45,7 @@ def lib():
             "orc_bproj_sbp_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i]),
             "orc_bproj_matched_f32": (_i, [_gp, _f32p, _i32p, _f32p, _f32p, _i, _i, _i]),
             "orc_bproj_matched_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i]),
-            "orc_bproj_clip_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i]),
+            "orc_bproj_clip_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _i, _i, _i, _i]),
             "orc_bp_init_f64": (_i, [_gp, _f32p, _i, _f64p, _f64p]),
             "orc_prior_f32": (_i, [_gp, _f32p, _f32p, _f32p, _i, _f, _f, _i]),
             "orc_prior_f64": (_i, [_gp, _f64p, _f64p, _f64p, _i, _d, _d, _i]),
@@ -59,9 +59,9 @@ def lib():
             "orc_regroup_f32": (None, [_f32p, _f32p, _i, _i, _i64]),
             "orc_objective_f64": (_d, [_gp, _f32p, _i, _f64p, _f64p, _f64p, _vp, _i, _d, _d, _i]),
             "orc_run_f32": (_i, [_gp, _f32p, _i, _f32p, _f32p, _vp, _vp, _i, _i, _d, _d, _i, _i,
-                                 _i, _i, _f32p, _vp]),
+                                 _i, _i, _i, _f32p, _vp]),
             "orc_run_f64": (_i, [_gp, _f32p, _i, _f64p, _f64p, _vp, _vp, _i, _i, _d, _d, _i, _i,
-                                 _i, _i, _f64p, _vp]),
+                                 _i, _i, _i, _f64p, _vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -146,12 +146,13 @@ def bproj_matched(g: Geometry, angles, prj, a0, a1, index=None, cos_weight=True,
     return img
 
 
-def bproj_clip(g: Geometry, angles, prj, a0, a1, index=None, cos_weight=True, img=None):
+def bproj_clip(g: Geometry, angles, prj, a0, a1, index=None, cos_weight=True, img=None,
+               nthreads=1):
     """img += exact geometric transpose (voxel-driven ray/box clipping), FP64."""
     img = np.zeros(g.vol_shape, np.float64) if img is None else _c(img, np.float64).copy()
     lib().orc_bproj_clip_f64(ctypes.byref(g.to_c()), _c(angles, np.float32),
                              _idx(index, len(angles)), img.ravel(), _c(prj, np.float64).ravel(),
-                             a0, a1, int(cos_weight))
+                             a0, a1, int(cos_weight), nthreads)
     return img
 
 
@@ -228,8 +229,10 @@ def objective(g: Geometry, angles, x, y, b, r=None, potential=2, beta=0.0, delta
 
 def run(g: Geometry, angles, y, b, r=None, x0=None, n_subsets=3, n_iters=10, beta=1e3,
         delta=5e-4, potential=2, gamma_mode=1, guard=True, dtype=np.float32, snapshots=False,
-        nthreads=0):
-    """Full OSTR driver (Tomo_OSTR_gold.cpp:301-918 order). Returns (x, snaps|None)."""
+        nthreads=0, backprojector=0):
+    """Full OSTR driver (Tomo_OSTR_gold.cpp:301-918 order). Returns (x, snaps|None).
+    backprojector 0 = the reference's SBP; 1 = the exact transpose of the forward
+    model (d = G^T(yry G 1), capL = M G^T h)."""
     nv = len(angles)
     y, b = _c(y, dtype), _c(b, dtype)
     rr = None if r is None else _c(r, dtype)
@@ -239,7 +242,7 @@ def run(g: Geometry, angles, y, b, r=None, x0=None, n_subsets=3, n_iters=10, bet
     rc = getattr(lib(), "orc_run_" + _dt(dtype))(
         ctypes.byref(g.to_c()), _c(angles, np.float32), nv, y.ravel(), b.ravel(), _ptr(rr),
         _ptr(xi), n_subsets, n_iters, beta, delta, int(potential), int(gamma_mode), int(guard),
-        nthreads, out.ravel(), _ptr(snaps))
+        int(backprojector), nthreads, out.ravel(), _ptr(snaps))
     if rc != 0:
         raise ValueError(f"oracle run: invalid configuration (code {rc})")
     return out, snaps

@@ -339,8 +339,11 @@ void backproject_matched(const Geo<T>& g, const float* angles, const int* index,
 // the exit.
 template <typename T>
 void backproject_clip(const Geo<T>& g, const Geo<float>& gf, const float* angles,
-                      const int* index, T* img, const T* prj, int a0, int a1, int cos_weight) {
+                      const int* index, T* img, const T* prj, int a0, int a1, int cos_weight,
+                      int nthr = 1) {
     const T ez = (T)(-gf.det_r), sy = (T)gf.src_y;
+    // one thread per slice; each voxel is summed by one thread in a fixed order
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthr > 0 ? nthr : 1)
     for (int iz = 0; iz < g.nz; ++iz)
         for (int iy = 0; iy < g.ny; ++iy)
             for (int ix = 0; ix < g.nx; ++ix) {
@@ -626,19 +629,28 @@ double objective(const Geo<T>& g, const float* angles, const int* index, int n_v
 }
 
 // ------------------------------------------------------------ driver ----
+// backprojector 0: the reference's SBP for d and capL (Tomo_OSTR_gold.cpp
+// :497, :759). 1: the exact transpose G^T of the forward model G = cos(t) A
+// (backproject_clip), with d = G^T(yry * G 1) - the separable-quadratic-
+// surrogate denominator for which the iteration is a descent method on the
+// objective (with SBP, B != G^T and it is not); gamma_mode is not used.
 struct RunOpts {
     int n_subsets, n_iters, potential, gamma_mode, guard, nthreads;
     double beta, delta;
+    int backprojector;
 };
 
 template <typename T>
 int run(const ostr_geom& gg, const float* angles, int n_views, const T* y_in, const T* b_in,
         const T* r_in, const T* x_init, const RunOpts& o, T* x_out, T* snapshots) {
     Geo<T> g(gg);
+    const Geo<float> gf(gg);
     const int M = o.n_subsets;
     if (M <= 0 || n_views % M != 0) return OSTR_E_ARG;
     if (g.dx != g.dy) return OSTR_E_ARG;
     if (o.potential != OSTR_PRIOR_Q2 && o.delta == 0) return OSTR_E_ARG;
+    if (o.backprojector != 0 && o.backprojector != 1) return OSTR_E_ARG;
+    const bool matched = o.backprojector == 1;
     const int vs = n_views / M;
     const int64_t nbin = g.nbin(), nvox = g.nvox(), nall = nbin * n_views;
     std::vector<int> index(n_views);
@@ -649,10 +661,20 @@ int run(const ostr_geom& gg, const float* angles, int n_views, const T* y_in, co
     // precompute: yry, gamma, gamma*yry, d (Tomo_OSTR_gold.cpp:415-560)
     std::vector<T> c(nall), gam(nall), d(nvox, (T)0);
     yry(c.data(), y_in, r_in, nall);
-    project(g, angles, index.data(), (const T*)nullptr, gam.data(), 0, n_views,
-            o.gamma_mode == OSTR_GAMMA_ROW_SUM ? 2 : 1, 0, o.nthreads);
+    if (matched) {  // gamma = G 1 (cos-weighted row sums)
+        std::vector<T> ones(nvox, (T)1);
+        project(g, angles, index.data(), ones.data(), gam.data(), 0, n_views, 0, 1,
+                o.nthreads);
+    } else {
+        project(g, angles, index.data(), (const T*)nullptr, gam.data(), 0, n_views,
+                o.gamma_mode == OSTR_GAMMA_ROW_SUM ? 2 : 1, 0, o.nthreads);
+    }
     for (int64_t i = 0; i < nall; ++i) gam[i] = c[i] * gam[i];
-    backproject_sbp(g, angles, index.data(), d.data(), gam.data(), 0, n_views, o.nthreads);
+    if (matched)
+        backproject_clip(g, gf, angles, index.data(), d.data(), gam.data(), 0, n_views, 1,
+                         o.nthreads);
+    else
+        backproject_sbp(g, angles, index.data(), d.data(), gam.data(), 0, n_views, o.nthreads);
 
     // regroup data and view order (:602-642)
     std::vector<T> y(nall), b(nall), r;
@@ -678,7 +700,12 @@ int run(const ostr_geom& gg, const float* angles, int n_views, const T* y_in, co
             compute_h(h.data(), y.data() + a0 * nbin, b.data() + a0 * nbin, l.data(),
                       r_in ? r.data() + a0 * nbin : (const T*)nullptr, vs * nbin);
             std::fill(capl.begin(), capl.end(), (T)0);
-            backproject_sbp(g, angles, idx.data(), capl.data(), h.data(), a0, a1, o.nthreads);
+            if (matched)
+                backproject_clip(g, gf, angles, idx.data(), capl.data(), h.data(), a0, a1, 1,
+                                 o.nthreads);
+            else
+                backproject_sbp(g, angles, idx.data(), capl.data(), h.data(), a0, a1,
+                                o.nthreads);
             std::fill(rd.begin(), rd.end(), (T)0);
             std::fill(rdd.begin(), rdd.end(), (T)0);
             prior(g, rdd.data(), rd.data(), x.data(), o.potential, delta, rho, o.nthreads);
@@ -768,8 +795,9 @@ ORC_EXPORT int orc_bp_init_f64(const ostr_geom* g, const float* ang, int nv, con
     return 0;
 }
 ORC_EXPORT int orc_bproj_clip_f64(const ostr_geom* g, const float* ang, const int* idx,
-                                  double* img, const double* prj, int a0, int a1, int cw) {
-    backproject_clip(Geo<double>(*g), Geo<float>(*g), ang, idx, img, prj, a0, a1, cw);
+                                  double* img, const double* prj, int a0, int a1, int cw,
+                                  int nt) {
+    backproject_clip(Geo<double>(*g), Geo<float>(*g), ang, idx, img, prj, a0, a1, cw, nt);
     return 0;
 }
 ORC_EXPORT int orc_bproj_matched_f64(const ostr_geom* g, const float* ang, const int* idx,
@@ -839,14 +867,16 @@ ORC_EXPORT double orc_objective_f64(const ostr_geom* g, const float* ang, int nv
 ORC_EXPORT int orc_run_f32(const ostr_geom* g, const float* ang, int nv, const float* y,
                            const float* b, const float* r, const float* x0, int n_subsets,
                            int n_iters, double beta, double delta, int pot, int gamma_mode,
-                           int guard, int nt, float* x_out, float* snapshots) {
-    RunOpts o{n_subsets, n_iters, pot, gamma_mode, guard, nt, beta, delta};
+                           int guard, int backprojector, int nt, float* x_out,
+                           float* snapshots) {
+    RunOpts o{n_subsets, n_iters, pot, gamma_mode, guard, nt, beta, delta, backprojector};
     return run<float>(*g, ang, nv, y, b, r, x0, o, x_out, snapshots);
 }
 ORC_EXPORT int orc_run_f64(const ostr_geom* g, const float* ang, int nv, const double* y,
                            const double* b, const double* r, const double* x0, int n_subsets,
                            int n_iters, double beta, double delta, int pot, int gamma_mode,
-                           int guard, int nt, double* x_out, double* snapshots) {
-    RunOpts o{n_subsets, n_iters, pot, gamma_mode, guard, nt, beta, delta};
+                           int guard, int backprojector, int nt, double* x_out,
+                           double* snapshots) {
+    RunOpts o{n_subsets, n_iters, pot, gamma_mode, guard, nt, beta, delta, backprojector};
     return run<double>(*g, ang, nv, y, b, r, x0, o, x_out, snapshots);
 }

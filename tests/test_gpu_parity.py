@@ -215,6 +215,30 @@ def test_ostr_small_per_iteration_parity(small_case, dev, fused, oracle_lib):
     assert max(errs) <= 1e-4, errs
 
 
+@pytest.mark.parametrize("M", [1, 3])
+def test_ostr_matched_per_iteration_parity(small_case, dev, M, oracle_lib):
+    """OS-SQS with the exact transpose (d = G^T(yry G 1), capL = M G^T h):
+    <= 1e-4 rel RMSE against the FP32 oracle (clip transpose) after every
+    iteration, and the FP64 objective decreases monotonically for M = 1."""
+    s, x, y, b = small_case
+    n = 6
+    cfg = OSTRConfig(n_subsets=M, beta=1e3, delta=5e-4, prior="hyperbola",
+                     backprojector="matched")
+    _, snaps = oracle_lib.run(s.geom, s.angles, y, b, n_subsets=M, n_iters=n, beta=1e3,
+                              delta=5e-4, potential=2, snapshots=True, backprojector=1)
+    rec = OSTR(s.geom, s.angles, y, b, cfg=cfg, device=dev)
+    assert not rec.fused
+    errs, objs = [], []
+
+    def cb(it, xg):
+        errs.append(rel_rmse(xg.cpu().numpy(), snaps[it - 1]))
+        objs.append(rec.objective())
+    rec.iterate(n, callback=cb)
+    assert max(errs) <= 1e-4, errs
+    if M == 1:
+        assert all(b2 <= b1 for b1, b2 in zip(objs, objs[1:])), objs
+
+
 def test_fused_equals_unfused(small_case, dev):
     s, x, y, b = small_case
     out = []

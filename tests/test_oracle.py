@@ -445,3 +445,52 @@ def test_invalid_configs_rejected(oracle_lib):
         oracle_lib.run(s.geom, s.angles, y, y, n_subsets=3, delta=0.0, potential=1)
     with pytest.raises(ValueError):
         oracle_lib.prior(s.geom, np.zeros(s.geom.vol_shape, np.float32), 2, 0.0)
+
+
+def _tiny_case(oracle_lib, seed=5):
+    from newrecon_amd.geometry import scan_tiny
+    from newrecon_amd.phantom import breast_phantom, poisson_counts
+    s = scan_tiny()
+    x = breast_phantom(s.geom, 3, (2, 3), 1).astype(np.float64)
+    l = oracle_lib.fproj(s.geom, s.angles, x, 0, s.n_views, dtype=np.float64)
+    y = poisson_counts(l, s.i0, seed).astype(np.float64)
+    return s, x, y, np.full_like(y, s.i0)
+
+
+def test_matched_update_direction_is_objective_gradient(oracle_lib):
+    """With the exact transpose, G^T h + beta RD is the gradient of the FP64
+    objective (data term + beta * penalty): directional finite differences
+    along random directions. An SBP-style or uncos-weighted transpose, or a
+    wrong sign of h, fails this."""
+    s, x, y, b = _tiny_case(oracle_lib)
+    g, beta, delta = s.geom, 1e3, 5e-4
+    x = x + 0.002  # keep every voxel off the x >= 0 bound
+    l = oracle_lib.fproj(g, s.angles, x, 0, s.n_views, dtype=np.float64)
+    h = oracle_lib.compute_h(y, b, l, dtype=np.float64)
+    grad = oracle_lib.bproj_clip(g, s.angles, h, 0, s.n_views, cos_weight=True)
+    _, rd = oracle_lib.prior(g, x, 2, delta, dtype=np.float64)
+    grad = grad + beta * rd
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        v = rng.standard_normal(x.shape)
+        eps = 1e-6
+        fp = oracle_lib.objective(g, s.angles, x + eps * v, y, b, potential=2, beta=beta,
+                                  delta=delta)
+        fm = oracle_lib.objective(g, s.angles, x - eps * v, y, b, potential=2, beta=beta,
+                                  delta=delta)
+        fd = (fp - fm) / (2 * eps)
+        an = float((grad * v).sum())
+        assert abs(fd - an) <= 1e-5 * abs(an) + 1e-3, (fd, an)
+
+
+def test_matched_sqs_decreases_objective(oracle_lib):
+    """M = 1 with the exact transpose is a separable-quadratic-surrogate
+    descent: the FP64 objective never increases."""
+    s, x, y, b = _tiny_case(oracle_lib, seed=6)
+    _, snaps = oracle_lib.run(s.geom, s.angles, y, b, n_subsets=1, n_iters=12, beta=1e3,
+                              delta=5e-4, potential=2, dtype=np.float64, snapshots=True,
+                              backprojector=1)
+    objs = [oracle_lib.objective(s.geom, s.angles, z, y, b, potential=2, beta=1e3, delta=5e-4)
+            for z in snaps]
+    assert all(o2 <= o1 + 1e-12 * abs(o1) for o1, o2 in zip(objs, objs[1:])), objs
+    assert objs[-1] < objs[0]
