@@ -227,6 +227,20 @@ int ostr_objective(ostr_handle* h, const void* img, int32_t img_dtype, const flo
 int ostr_diff_norms(ostr_handle* h, const void* a, const void* b, int32_t dtype, int64_t n,
                     double* out, void* stream);
 
+/* x = max(0, z + c (z - z_prev)) over the volume (f32): the momentum step of
+ * accelerated OSTR (Nesterov momentum on the OS / SQS update, Kim, Ramani &
+ * Fessler, IEEE TMI 2015). No reference counterpart (the reference iterates
+ * plain OSTR only). x may alias neither z nor z_prev. */
+int ostr_extrapolate(ostr_handle* h, float* x, const float* z, const float* z_prev, float c,
+                     void* stream);
+
+/* Synchronous. *out = sum over the volume of (x - z)(z - z_prev) in FP64
+ * (deterministic): the gradient-restart test of accelerated SQS
+ * (O'Donoghue & Candes 2015): restart the momentum when it is > 0, x being
+ * the point the last pass started from and z its result. */
+int ostr_restart_dot(ostr_handle* h, const float* x, const float* z, const float* z_prev,
+                     double* out, void* stream);
+
 /* ---- batched cases: one launch over several reconstructions ----------------
  * The reference runs one geometry per process (__constant__ state,
  * Tomo_OSTR.cu:81-115) and so one case per launch; these calls have no

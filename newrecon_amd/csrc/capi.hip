@@ -459,6 +459,37 @@ OSTR_API int ostr_update_est(ostr_handle* h, float* est, const float* capL, cons
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_update_est");
 }
 
+OSTR_API int ostr_extrapolate(ostr_handle* h, float* x, const float* z, const float* z_prev,
+                              float c, void* stream) {
+    CHECK_HANDLE(h);
+    if (!x || !z || !z_prev) return fail(h, OSTR_E_ARG, "ostr_extrapolate: NULL buffer");
+    if (x == z || x == z_prev) return fail(h, OSTR_E_ARG, "ostr_extrapolate: x aliases an input");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    int64_t n = (int64_t)h->geom.nx * h->geom.ny * h->geom.nz;
+    hipError_t e = ostr::launch_extrapolate(x, z, z_prev, c, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_extrapolate");
+}
+
+OSTR_API int ostr_restart_dot(ostr_handle* h, const float* x, const float* z,
+                              const float* z_prev, double* out, void* stream) {
+    CHECK_HANDLE(h);
+    if (!x || !z || !z_prev || !out) return fail(h, OSTR_E_ARG, "ostr_restart_dot: NULL buffer");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = ensure_dwork(h, 1024 + 1, st, "ostr_restart_dot"))) return rc;
+    int64_t n = (int64_t)h->geom.nx * h->geom.ny * h->geom.nz;
+    hipError_t e = ostr::launch_restart_dot(x, z, z_prev, n, h->dwork, h->dwork + 1024, st);
+    double host = 0.0;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&host, h->dwork + 1024, sizeof host, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(h, e, "ostr_restart_dot");
+    *out = host;
+    return OSTR_OK;
+}
+
 OSTR_API int ostr_regroup(ostr_handle* h, float* dst, const float* src, int32_t n_subsets,
                           int32_t to_rows, void* stream) {
     CHECK_HANDLE(h);
@@ -886,4 +917,162 @@ OSTR_API int ostr_batch_update_fused(ostr_batch* bt, const ostr_batch_case* case
     hipError_t e = ostr::launch_batch_update(table, sched, n_cases, total, grid, cfg->upd_rows,
                                              cfg->upd_waves, potential, prj_dtype, st);
     return batch_finish(bt, slot, e, st, "ostr_batch_update_fused");
+}
+
+// ---- warm-start network kernels (include/ostr_cnn.h) ------------------------
+
+#include "../../include/ostr_cnn.h"
+
+namespace {
+
+int cnn_fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int cnn_done(hipError_t e, const char* fn) {
+    if (e == hipSuccess) return OSTR_OK;
+    return cnn_fail(OSTR_E_HIP, "%s: HIP error %d (%s)", fn, (int)e, hipGetErrorString(e));
+}
+
+bool dims_ok(int nz, int ny, int nx) { return nz > 0 && ny > 0 && nx > 0; }
+
+}  // namespace
+
+OSTR_API int32_t ostr_cnn_kpad(int32_t c) { return ostr::cnn_kpad(c); }
+
+OSTR_API int ostr_cnn_pack(const float* w, int32_t cin, int32_t cout, void* wf, void* wd,
+                           void* stream) {
+    if (!w || !wf || !wd || cin <= 0 || cout <= 0)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_pack: bad argument");
+    return cnn_done(ostr::launch_cnn_pack(w, cin, cout, wf, wd, (hipStream_t)stream),
+                    "ostr_cnn_pack");
+}
+
+OSTR_API int ostr_cnn_conv3d(int32_t cin, int32_t cout, const void* x, int32_t xcs, int32_t xco,
+                             const void* wf, const float* bias, void* y, int32_t ycs,
+                             int32_t yco, const void* mask, int32_t mcs, int32_t mco,
+                             int32_t relu, int32_t nz, int32_t ny, int32_t nx, void* stream) {
+    if (!ostr::cnn_conv_supported(cin, cout))
+        return cnn_fail(OSTR_E_UNSUPPORTED, "ostr_cnn_conv3d: shape %dx%d not compiled", cin,
+                        cout);
+    if (!x || !wf || !y || !dims_ok(nz, ny, nx))
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_conv3d: bad argument");
+    if (xcs % 8 || xco % 8 || xco + cin > xcs || ycs % 4 || yco % 4 || yco + cout > ycs ||
+        (mask && (mcs % 4 || mco % 4 || mco + cout > mcs)))
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_conv3d: channel stride/offset not aligned or "
+                                    "out of range");
+    return cnn_done(ostr::launch_cnn_conv(cin, cout, x, xcs, xco, wf, bias, y, ycs, yco, mask,
+                                          mcs, mco, relu, nz, ny, nx, (hipStream_t)stream),
+                    "ostr_cnn_conv3d");
+}
+
+OSTR_API size_t ostr_cnn_wgrad_workspace(int32_t cin, int32_t cout, int32_t nz, int32_t ny,
+                                         int32_t nx, int32_t nsplit) {
+    return ostr::cnn_wgrad_work_bytes(cin, cout, nz, ny, nx, nsplit);
+}
+
+OSTR_API int ostr_cnn_wgrad(int32_t cin, int32_t cout, const void* x, int32_t xcs, int32_t xco,
+                            const void* dy, int32_t dcs, int32_t dco, int32_t nz, int32_t ny,
+                            int32_t nx, float* dw, float* db, void* work, size_t work_bytes,
+                            int32_t nsplit, void* stream) {
+    if (!ostr::cnn_wgrad_supported(cin, cout))
+        return cnn_fail(OSTR_E_UNSUPPORTED, "ostr_cnn_wgrad: shape %dx%d not compiled", cin,
+                        cout);
+    if (!x || !dy || !dw || !work || !dims_ok(nz, ny, nx) || nsplit < 1 || nsplit > 1024)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_wgrad: bad argument");
+    if (xco + cin > xcs || dco + cout > dcs)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_wgrad: channel view out of range");
+    if (work_bytes < ostr::cnn_wgrad_work_bytes(cin, cout, nz, ny, nx, nsplit))
+        return cnn_fail(OSTR_E_RANGE, "ostr_cnn_wgrad: workspace too small");
+    return cnn_done(ostr::launch_cnn_wgrad(cin, cout, x, xcs, xco, dy, dcs, dco, nz, ny, nx, dw,
+                                           db, work, nsplit, (hipStream_t)stream),
+                    "ostr_cnn_wgrad");
+}
+
+OSTR_API int ostr_cnn_pool2(const void* x, int32_t xcs, int32_t xco, void* y, int32_t ycs,
+                            int32_t yco, int32_t C, int32_t nz, int32_t ny, int32_t nx,
+                            void* stream) {
+    if (!x || !y || C <= 0 || !dims_ok(nz, ny, nx) || nz % 2 || ny % 2 || nx % 2)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_pool2: bad argument (sizes must be even)");
+    return cnn_done(ostr::launch_cnn_pool(x, xcs, xco, y, ycs, yco, C, nz, ny, nx,
+                                          (hipStream_t)stream),
+                    "ostr_cnn_pool2");
+}
+
+OSTR_API int ostr_cnn_pool2_bwd(const void* dy, int32_t dcs, int32_t dco, void* dx, int32_t xcs,
+                                int32_t xco, const void* mask, int32_t mcs, int32_t mco,
+                                int32_t accumulate, int32_t C, int32_t nz, int32_t ny,
+                                int32_t nx, void* stream) {
+    if (!dy || !dx || C <= 0 || !dims_ok(nz, ny, nx) || nz % 2 || ny % 2 || nx % 2)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_pool2_bwd: bad argument (sizes must be even)");
+    return cnn_done(ostr::launch_cnn_pool_bwd(dy, dcs, dco, dx, xcs, xco, mask, mcs, mco,
+                                              accumulate, C, nz, ny, nx, (hipStream_t)stream),
+                    "ostr_cnn_pool2_bwd");
+}
+
+OSTR_API int ostr_cnn_up2(const void* x, int32_t xcs, int32_t xco, void* y, int32_t ycs,
+                          int32_t yco, int32_t C, int32_t nz, int32_t ny, int32_t nx,
+                          void* stream) {
+    if (!x || !y || C <= 0 || !dims_ok(nz, ny, nx) || nz % 2 || ny % 2 || nx % 2)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_up2: bad argument (sizes must be even)");
+    return cnn_done(ostr::launch_cnn_up(x, xcs, xco, y, ycs, yco, C, nz, ny, nx,
+                                        (hipStream_t)stream),
+                    "ostr_cnn_up2");
+}
+
+OSTR_API int ostr_cnn_up2_bwd(const void* dy, int32_t dcs, int32_t dco, void* dx, int32_t xcs,
+                              int32_t xco, const void* mask, int32_t mcs, int32_t mco, int32_t C,
+                              int32_t nz, int32_t ny, int32_t nx, void* stream) {
+    if (!dy || !dx || C <= 0 || !dims_ok(nz, ny, nx) || nz % 2 || ny % 2 || nx % 2)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_up2_bwd: bad argument (sizes must be even)");
+    return cnn_done(ostr::launch_cnn_up_bwd(dy, dcs, dco, dx, xcs, xco, mask, mcs, mco, C, nz,
+                                            ny, nx, (hipStream_t)stream),
+                    "ostr_cnn_up2_bwd");
+}
+
+OSTR_API int ostr_cnn_input(const float* img, int64_t n, float scale, void* x, void* stream) {
+    if (!img || !x || n <= 0) return cnn_fail(OSTR_E_ARG, "ostr_cnn_input: bad argument");
+    return cnn_done(ostr::launch_cnn_input(img, n, scale, x, (hipStream_t)stream),
+                    "ostr_cnn_input");
+}
+
+OSTR_API int ostr_cnn_output(const void* out, int32_t ocs, const float* img, int64_t n,
+                             float scale, int32_t clamp, float* pred, void* stream) {
+    if (!out || !img || !pred || n <= 0 || ocs <= 0 || scale == 0.0f)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_output: bad argument");
+    return cnn_done(ostr::launch_cnn_output(out, ocs, img, n, 1.0f / scale, clamp, pred,
+                                            (hipStream_t)stream),
+                    "ostr_cnn_output");
+}
+
+OSTR_API int64_t ostr_cnn_mse_workspace(int64_t n) { return ostr::cnn_mse_partials(n) + 1; }
+
+OSTR_API int ostr_cnn_mse(const float* pred, const float* target, int64_t n, float gscale,
+                          void* dout, int32_t dcs, double* work, int64_t work_elems,
+                          double* loss, void* stream) {
+    if (!pred || !target || !work || !loss || n <= 0 || (dout && (dcs <= 0 || dcs % 8)))
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_mse: bad argument");
+    const int64_t np = ostr::cnn_mse_partials(n);
+    if (work_elems < np + 1) return cnn_fail(OSTR_E_RANGE, "ostr_cnn_mse: workspace too small");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = ostr::launch_cnn_mse(pred, target, n, gscale, dout, dcs, work, work + np, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(loss, work + np, sizeof(double),
+                                            hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return cnn_done(e, "ostr_cnn_mse");
+}
+
+OSTR_API int ostr_cnn_adam(float* p, float* m, float* v, const float* g, int64_t n, float lr,
+                           float b1, float b2, float eps, int32_t step, void* stream) {
+    if (!p || !m || !v || !g || n <= 0 || step < 1)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_adam: bad argument");
+    return cnn_done(ostr::launch_cnn_adam(p, m, v, g, n, lr, b1, b2, eps, step,
+                                          (hipStream_t)stream),
+                    "ostr_cnn_adam");
 }

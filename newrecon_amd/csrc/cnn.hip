@@ -1,0 +1,581 @@
+// cnn.hip - kernels of the warm-start network: 3-D convolution (forward,
+// data gradient, weight gradient) on BF16 MFMA with FP32 accumulation,
+// 2x average pooling and 2x nearest upsampling with their gradients, the
+// input/output/MSE head, weight packing and Adam.
+//
+// Tensors are channels-last BF16 views (pointer, channel stride, channel
+// offset): element (voxel v, channel c) at p[v * cs + co + c], voxel
+// v = (z * ny + y) * nx + x. Skip connections are free: a layer writes into
+// its channel range of the concatenation buffer the next layer reads.
+//
+// Convolution (3x3x3, zero padding 1, stride 1) as an implicit GEMM on
+// v_mfma_f32_16x16x32_bf16 (A 16x32, B 32x16, D 16x16; lane l holds A row
+// l%16 / B column l%16 at k = 8*(l/16)+0..7, and D column l%16 at rows
+// 4*(l/16)+0..3):
+//   D[co][voxel] += sum_k W[co][k] * P[k][voxel],  k = tap * CIN + ci,
+//   tap = (dz+1)*9 + (dy+1)*3 + (dx+1),  P[k][v] = X[v + (dz,dy,dx)][ci].
+// A is read as 16 bytes of the packed weights, B as 16 bytes (8 channels of
+// one tap) of the input, and a lane stores 4 consecutive output channels of
+// one voxel (8 bytes). The data gradient is the same kernel on the output
+// gradient with the flipped, transposed weights, masked by the layer input's
+// ReLU (dX = 0 where X <= 0).
+//
+// Weight gradient: dW[co][tap][ci] = sum_v dY[v][co] X[v + off(tap)][ci],
+// a GEMM reduced over voxels: A = dY (16 co x 32 voxels), B = X shifted by
+// the tap (32 voxels x 16 ci), both read 16 bytes at a time from
+// channel-major, row-padded copies [C][nz][ny][nxp] (three x-shifted copies
+// of X, so every load is aligned). Each work-group reduces a share of the
+// rows for one tap; its waves' sums are combined in LDS in a fixed order and
+// written as a partial; a second kernel sums the partials in a fixed order,
+// so the gradient is deterministic.
+#include "ostr_internal.h"
+
+namespace ostr {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+constexpr int kpad(int cin) { return ((27 * cin + 31) / 32) * 32; }
+
+__device__ __forceinline__ u16 to_bf16(float v) { return bf16_bits(v); }
+__device__ __forceinline__ float from_bf16(u16 b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ bf16x8 load8(const u16* p) {
+    return *reinterpret_cast<const bf16x8*>(p);
+}
+__device__ __forceinline__ bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) z[e] = (__bf16)0.0f;
+    return z;
+}
+
+// ---- convolution (forward and data gradient) -------------------------------
+// Work-group: 4 waves x NVW voxels along x of one row (y, z).
+template <int CIN, int COUT, int NVW>
+__global__ __launch_bounds__(256) void conv3d_kernel(
+    const u16* __restrict__ x, int xcs, int xco, const u16* __restrict__ w,
+    const float* __restrict__ bias, u16* __restrict__ y, int ycs, int yco,
+    const u16* __restrict__ mask, int mcs, int mco, int relu, int nz, int ny, int nx) {
+    constexpr int KP = kpad(CIN), NCHUNK = KP / 32, NCO = COUT / 16, NVB = NVW / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tiles_x = (nx + 4 * NVW - 1) / (4 * NVW);
+    int b = blockIdx.x;
+    const int tx = b % tiles_x;
+    b /= tiles_x;
+    const int iy = b % ny, iz = b / ny;
+    const int xw = tx * 4 * NVW + wave * NVW;
+    if (xw >= nx) return;  // whole wave past the row (no cross-wave sync)
+    const int j = lane & 15, kg = lane >> 4;
+    f32x4 acc[NCO][NVB];
+#pragma unroll
+    for (int cb = 0; cb < NCO; ++cb)
+#pragma unroll
+        for (int nb = 0; nb < NVB; ++nb) acc[cb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ch = 0; ch < NCHUNK; ++ch) {
+        const int k0 = ch * 32 + kg * 8;
+        const int tap = k0 / CIN, ci = k0 % CIN;  // 8 k of one tap (CIN % 8 == 0)
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const int zz = iz + dz, yy = iy + dy;
+        const bool row_ok = tap < 27 && (unsigned)zz < (unsigned)nz && (unsigned)yy < (unsigned)ny;
+        bf16x8 bop[NVB];
+#pragma unroll
+        for (int nb = 0; nb < NVB; ++nb) {
+            const int xx = xw + nb * 16 + j + dx;
+            if (row_ok && (unsigned)xx < (unsigned)nx)
+                bop[nb] = load8(x + (((int64_t)zz * ny + yy) * nx + xx) * xcs + xco + ci);
+            else
+                bop[nb] = zero8();
+        }
+#pragma unroll
+        for (int cb = 0; cb < NCO; ++cb) {
+            const bf16x8 aop = load8(w + (int64_t)(cb * 16 + j) * KP + k0);
+#pragma unroll
+            for (int nb = 0; nb < NVB; ++nb)
+                acc[cb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop, bop[nb], acc[cb][nb],
+                                                                      0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NVB; ++nb) {
+        const int xx = xw + nb * 16 + j;
+        if (xx >= nx) continue;
+        const int64_t v = ((int64_t)iz * ny + iy) * nx + xx;
+#pragma unroll
+        for (int cb = 0; cb < NCO; ++cb) {
+            const int c0 = cb * 16 + kg * 4;
+            u16 o[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float val = acc[cb][nb][r];
+                if (bias) val += bias[c0 + r];
+                if (relu) val = val > 0.0f ? val : 0.0f;
+                if (mask && !(from_bf16(mask[v * mcs + mco + c0 + r]) > 0.0f)) val = 0.0f;
+                o[r] = to_bf16(val);
+            }
+            uint2 packed;
+            packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+            packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+            *reinterpret_cast<uint2*>(y + v * ycs + yco + c0) = packed;
+        }
+    }
+}
+
+// ---- weight gradient -----------------------------------------------------
+// Channel-major, row-padded copy [C16][nz][ny][nxp] (C16 = C rounded up to
+// 16, extra channels zero) of a channels-last view, shifted by dx along x
+// (element x holds voxel x + dx, zero outside [0, nx) and for x >= nx).
+__global__ void to_cm_kernel(const u16* __restrict__ src, int cs, int co, int C, int C16,
+                             u16* __restrict__ dst, int nz, int ny, int nx, int nxp, int dx) {
+    const int64_t n = (int64_t)C16 * nz * ny * nxp;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int xx = (int)(i % nxp);
+        int64_t r = i / nxp;
+        const int row = (int)(r % ((int64_t)nz * ny));
+        const int c = (int)(r / ((int64_t)nz * ny));
+        const int xs = xx + dx;
+        u16 val = 0;
+        if (c < C && xx < nx && (unsigned)xs < (unsigned)nx)
+            val = src[((int64_t)row * nx + xs) * cs + co + c];
+        dst[i] = val;
+    }
+}
+
+// Partial weight gradient of tap `blockIdx.y` over the rows of split
+// blockIdx.x: part[split][co][tap * CIN + ci]. dy_cm: [COUT16][rows][nxp];
+// xs_cm: 3 shifted copies [3][CIN16][rows][nxp] (copy s holds dx = s - 1).
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void wgrad_kernel(const u16* __restrict__ dy_cm,
+                                                    const u16* __restrict__ xs_cm,
+                                                    float* __restrict__ part, int nz, int ny,
+                                                    int nxp, int rows_per_split) {
+    constexpr int CI16 = (CIN + 15) / 16 * 16, NCI = CI16 / 16, NCO = COUT / 16;
+    constexpr int K = 27 * CIN;
+    __shared__ float red[4][NCO * NCI * 4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, kg = lane >> 4;
+    const int tap = blockIdx.y, split = blockIdx.x;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int64_t rows = (int64_t)nz * ny, plane = rows * nxp;
+    const u16* xs = xs_cm + (int64_t)(dx + 1) * CI16 * plane;
+    f32x4 acc[NCO][NCI];
+#pragma unroll
+    for (int a = 0; a < NCO; ++a)
+#pragma unroll
+        for (int c = 0; c < NCI; ++c) acc[a][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int r0 = split * rows_per_split;
+    const int r1 = (int)((int64_t)r0 + rows_per_split < rows ? r0 + rows_per_split : rows);
+    const int xchunks = nxp / 32;
+    // chunk q = (row, 32-voxel segment); wave w takes q = w, w + 4, ...
+    for (int q = wave; q < (r1 - r0) * xchunks; q += 4) {
+        const int row = r0 + q / xchunks, x0 = (q % xchunks) * 32;
+        const int iz = row / ny, iy = row % ny;
+        const int zz = iz + dz, yy = iy + dy;
+        if ((unsigned)zz >= (unsigned)nz || (unsigned)yy >= (unsigned)ny) continue;
+        const int64_t rs = (int64_t)zz * ny + yy;
+        bf16x8 aop[NCO], bop[NCI];
+#pragma unroll
+        for (int a = 0; a < NCO; ++a)
+            aop[a] = load8(dy_cm + ((int64_t)(a * 16 + j) * rows + row) * nxp + x0 + kg * 8);
+#pragma unroll
+        for (int c = 0; c < NCI; ++c)
+            bop[c] = load8(xs + ((int64_t)(c * 16 + j) * rows + rs) * nxp + x0 + kg * 8);
+#pragma unroll
+        for (int a = 0; a < NCO; ++a)
+#pragma unroll
+            for (int c = 0; c < NCI; ++c)
+                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop[a], bop[c], acc[a][c], 0,
+                                                                    0, 0);
+    }
+    // combine the 4 waves in a fixed order
+#pragma unroll
+    for (int a = 0; a < NCO; ++a)
+#pragma unroll
+        for (int c = 0; c < NCI; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) red[wave][(a * NCI + c) * 4 + r][lane] = acc[a][c][r];
+    __syncthreads();
+    float* out = part + (int64_t)split * COUT * K;
+    for (int e = threadIdx.x; e < NCO * NCI * 4 * 64; e += 256) {
+        const int slot = e / 64, l = e % 64;
+        const float s = ((red[0][slot][l] + red[1][slot][l]) + red[2][slot][l]) + red[3][slot][l];
+        const int a = slot / (NCI * 4), c = (slot / 4) % NCI, r = slot % 4;
+        const int co = a * 16 + (l >> 4) * 4 + r;  // D row
+        const int ci = c * 16 + (l & 15);          // D column
+        if (ci < CIN) out[(int64_t)co * K + tap * CIN + ci] = s;
+    }
+}
+
+// dw[i] = sum_s part[s][i] (s ascending), i < n
+__global__ void sum_parts_kernel(const float* __restrict__ part, int nsplit, int64_t n,
+                                 float* __restrict__ dw) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int k = 0; k < nsplit; ++k) s += part[(int64_t)k * n + i];
+        dw[i] = s;
+    }
+}
+
+// db[co] = sum over voxels of dy (channel-major copy; padding is zero), one
+// work-group per channel, fixed-order reduction.
+__global__ __launch_bounds__(256) void bias_grad_kernel(const u16* __restrict__ dy_cm,
+                                                        int64_t per_channel,
+                                                        float* __restrict__ db) {
+    const u16* p = dy_cm + (int64_t)blockIdx.x * per_channel;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < per_channel; i += 256) s += from_bf16(p[i]);
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) db[blockIdx.x] = (float)s;
+}
+
+// ---- pooling / upsampling (channels-last views) ---------------------------
+// y (nz/2, ny/2, nx/2) = mean of the 8 children of x (nz, ny, nx).
+__global__ void pool_kernel(const u16* __restrict__ x, int xcs, int xco, u16* __restrict__ y,
+                            int ycs, int yco, int C, int nz, int ny, int nx) {
+    const int hz = nz / 2, hy = ny / 2, hx = nx / 2;
+    const int64_t n = (int64_t)hz * hy * hx * C;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const int64_t p = i / C;
+        const int px = (int)(p % hx), py = (int)((p / hx) % hy), pz = (int)(p / ((int64_t)hx * hy));
+        float s = 0.0f;
+        for (int q = 0; q < 8; ++q) {
+            const int64_t v = ((int64_t)(2 * pz + (q >> 2)) * ny + 2 * py + ((q >> 1) & 1)) * nx +
+                              2 * px + (q & 1);
+            s += from_bf16(x[v * xcs + xco + c]);
+        }
+        y[p * ycs + yco + c] = to_bf16(s * 0.125f);
+    }
+}
+
+// dx (nz, ny, nx) += dy(parent) / 8 where mask(child) > 0 (else dx unchanged
+// plus 0); accumulate = 0 overwrites dx instead.
+__global__ void pool_bwd_kernel(const u16* __restrict__ dy, int dcs, int dco,
+                                u16* __restrict__ dxp, int xcs, int xco,
+                                const u16* __restrict__ mask, int mcs, int mco, int accumulate,
+                                int C, int nz, int ny, int nx) {
+    const int64_t n = (int64_t)nz * ny * nx * C;
+    const int hy = ny / 2, hx = nx / 2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const int64_t v = i / C;
+        const int xx = (int)(v % nx), yy = (int)((v / nx) % ny), zz = (int)(v / ((int64_t)nx * ny));
+        const int64_t p = ((int64_t)(zz / 2) * hy + yy / 2) * hx + xx / 2;
+        float g = from_bf16(dy[p * dcs + dco + c]) * 0.125f;
+        if (mask && !(from_bf16(mask[v * mcs + mco + c]) > 0.0f)) g = 0.0f;
+        const int64_t o = v * xcs + xco + c;
+        dxp[o] = to_bf16(accumulate ? from_bf16(dxp[o]) + g : g);
+    }
+}
+
+// y (nz, ny, nx) = x (parent)
+__global__ void up_kernel(const u16* __restrict__ x, int xcs, int xco, u16* __restrict__ y,
+                          int ycs, int yco, int C, int nz, int ny, int nx) {
+    const int64_t n = (int64_t)nz * ny * nx * C;
+    const int hy = ny / 2, hx = nx / 2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const int64_t v = i / C;
+        const int xx = (int)(v % nx), yy = (int)((v / nx) % ny), zz = (int)(v / ((int64_t)nx * ny));
+        const int64_t p = ((int64_t)(zz / 2) * hy + yy / 2) * hx + xx / 2;
+        y[v * ycs + yco + c] = x[p * xcs + xco + c];
+    }
+}
+
+// dx (parent) = sum of dy over the 8 children, masked by mask(parent) > 0
+__global__ void up_bwd_kernel(const u16* __restrict__ dy, int dcs, int dco, u16* __restrict__ dx,
+                              int xcs, int xco, const u16* __restrict__ mask, int mcs, int mco,
+                              int C, int nz, int ny, int nx) {
+    const int hz = nz / 2, hy = ny / 2, hx = nx / 2;
+    const int64_t n = (int64_t)hz * hy * hx * C;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const int64_t p = i / C;
+        const int px = (int)(p % hx), py = (int)((p / hx) % hy), pz = (int)(p / ((int64_t)hx * hy));
+        float s = 0.0f;
+        for (int q = 0; q < 8; ++q) {
+            const int64_t v = ((int64_t)(2 * pz + (q >> 2)) * ny + 2 * py + ((q >> 1) & 1)) * nx +
+                              2 * px + (q & 1);
+            s += from_bf16(dy[v * dcs + dco + c]);
+        }
+        if (mask && !(from_bf16(mask[p * mcs + mco + c]) > 0.0f)) s = 0.0f;
+        dx[p * xcs + xco + c] = to_bf16(s);
+    }
+}
+
+// ---- head: input, output, loss -------------------------------------------
+// x[v][0..7] = (img[v] * scale, 0, ..., 0)
+__global__ void input_kernel(const float* __restrict__ img, int64_t n, float scale,
+                             u16* __restrict__ x) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        uint4 o = {0u, 0u, 0u, 0u};
+        o.x = to_bf16(img[v] * scale);
+        *reinterpret_cast<uint4*>(x + v * 8) = o;
+    }
+}
+
+// pred[v] = img[v] + out[v][0] / scale (residual); clamp at 0 when clamp != 0
+__global__ void output_kernel(const u16* __restrict__ out, int ocs, const float* __restrict__ img,
+                              int64_t n, float inv_scale, int clamp, float* __restrict__ pred) {
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        float p = img[v] + from_bf16(out[v * ocs]) * inv_scale;
+        pred[v] = (clamp && p < 0.0f) ? 0.0f : p;
+    }
+}
+
+// squared-error partial sums (FP64, one per work-group) and the gradient of
+// mean((pred - target)^2) with respect to out[v][0] (other channels 0).
+__global__ __launch_bounds__(256) void mse_kernel(const float* __restrict__ pred,
+                                                  const float* __restrict__ target, int64_t n,
+                                                  float gscale, u16* __restrict__ dout, int dcs,
+                                                  double* __restrict__ partials) {
+    double s = 0.0;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const float e = pred[v] - target[v];
+        s += (double)e * (double)e;
+        if (dout) {
+            uint4 z = {0u, 0u, 0u, 0u};
+            for (int c = 0; c < dcs; c += 8) *reinterpret_cast<uint4*>(dout + v * dcs + c) = z;
+            dout[v * dcs] = to_bf16(gscale * e);
+        }
+    }
+    s = block_sum_256(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// ---- parameters ------------------------------------------------------------
+// W (fp32 [COUT][27][CIN]) -> forward bf16 [COUT][kpad(CIN)] (k = tap*CIN+ci)
+// and data-gradient bf16 [CIN][kpad(COUT)] (k = tap*COUT+co, flipped taps).
+__global__ void pack_kernel(const float* __restrict__ wsrc, int cin, int cout,
+                            u16* __restrict__ wf, u16* __restrict__ wd) {
+    const int kpf = kpad(cin), kpd = kpad(cout);
+    const int64_t nf = (int64_t)cout * kpf, nd = (int64_t)cin * kpd;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nf + nd;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < nf) {
+            const int co = (int)(i / kpf), k = (int)(i % kpf);
+            wf[i] = k < 27 * cin ? to_bf16(wsrc[((int64_t)co * 27 + k / cin) * cin + k % cin]) : 0;
+        } else {
+            const int64_t e = i - nf;
+            const int ci = (int)(e / kpd), k = (int)(e % kpd);
+            u16 val = 0;
+            if (k < 27 * cout) {
+                const int tap = k / cout, co = k % cout;
+                val = to_bf16(wsrc[((int64_t)co * 27 + (26 - tap)) * cin + ci]);
+            }
+            wd[e] = val;
+        }
+    }
+}
+
+// Adam (Kingma & Ba 2015), bias-corrected, FP32 master parameters.
+__global__ void adam_kernel(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                            const float* __restrict__ g, int64_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+unsigned grid_for(int64_t n, int per = 256) {
+    int64_t g = (n + per - 1) / per;
+    if (g > 65536) g = 65536;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+template <int CIN, int COUT>
+hipError_t conv_t(const u16* x, int xcs, int xco, const u16* w, const float* bias, u16* y,
+                  int ycs, int yco, const u16* mask, int mcs, int mco, int relu, int nz, int ny,
+                  int nx, hipStream_t st) {
+    constexpr int NVW = COUT >= 64 ? 16 : 32;
+    const int64_t tiles_x = (nx + 4 * NVW - 1) / (4 * NVW);
+    const int64_t g = tiles_x * ny * nz;
+    hipLaunchKernelGGL((conv3d_kernel<CIN, COUT, NVW>), dim3((unsigned)g), dim3(256), 0, st, x,
+                       xcs, xco, w, bias, y, ycs, yco, mask, mcs, mco, relu, nz, ny, nx);
+    return hipGetLastError();
+}
+
+template <int CIN, int COUT>
+hipError_t wgrad_t(const u16* dy_cm, const u16* xs_cm, float* part, int nsplit, int nz, int ny,
+                   int nxp, hipStream_t st) {
+    const int rows = nz * ny;
+    const int per = (rows + nsplit - 1) / nsplit;
+    hipLaunchKernelGGL((wgrad_kernel<CIN, COUT>), dim3(nsplit, 27), dim3(256), 0, st, dy_cm,
+                       xs_cm, part, nz, ny, nxp, per);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Supported (CIN, COUT) pairs of the forward and data-gradient convolutions.
+#define OSTR_CNN_CONV_SHAPES(X) \
+    X(8, 16) X(16, 16) X(16, 32) X(32, 32) X(32, 64) X(64, 64) X(96, 32) X(48, 16) \
+    X(32, 16) X(64, 32) X(32, 96) X(16, 48)
+
+hipError_t launch_cnn_conv(int cin, int cout, const void* x, int xcs, int xco, const void* w,
+                           const float* bias, void* y, int ycs, int yco, const void* mask,
+                           int mcs, int mco, int relu, int nz, int ny, int nx, hipStream_t st) {
+#define CASE(A, B)                                                                          \
+    if (cin == A && cout == B)                                                              \
+        return conv_t<A, B>((const u16*)x, xcs, xco, (const u16*)w, bias, (u16*)y, ycs, yco, \
+                            (const u16*)mask, mcs, mco, relu, nz, ny, nx, st);
+    OSTR_CNN_CONV_SHAPES(CASE)
+#undef CASE
+    return hipErrorInvalidValue;
+}
+
+bool cnn_conv_supported(int cin, int cout) {
+#define CASE(A, B) if (cin == A && cout == B) return true;
+    OSTR_CNN_CONV_SHAPES(CASE)
+#undef CASE
+    return false;
+}
+
+// Supported (CIN, COUT) pairs of the weight gradient (forward shapes).
+#define OSTR_CNN_WGRAD_SHAPES(X) \
+    X(8, 16) X(16, 16) X(16, 32) X(32, 32) X(32, 64) X(64, 64) X(96, 32) X(48, 16)
+
+bool cnn_wgrad_supported(int cin, int cout) {
+#define CASE(A, B) if (cin == A && cout == B) return true;
+    OSTR_CNN_WGRAD_SHAPES(CASE)
+#undef CASE
+    return false;
+}
+
+int cnn_nxp(int nx) { return (nx + 31) / 32 * 32; }
+
+size_t cnn_wgrad_work_bytes(int cin, int cout, int nz, int ny, int nx, int nsplit) {
+    const int64_t plane = (int64_t)nz * ny * cnn_nxp(nx);
+    const int64_t ci16 = (cin + 15) / 16 * 16;
+    return sizeof(u16) * (size_t)(plane * cout + 3 * plane * ci16) +
+           sizeof(float) * (size_t)nsplit * cout * 27 * cin + 256;
+}
+
+hipError_t launch_cnn_wgrad(int cin, int cout, const void* x, int xcs, int xco, const void* dy,
+                            int dcs, int dco, int nz, int ny, int nx, float* dw, float* db,
+                            void* work, int nsplit, hipStream_t st) {
+    const int nxp = cnn_nxp(nx);
+    const int64_t plane = (int64_t)nz * ny * nxp;
+    const int ci16 = (cin + 15) / 16 * 16;
+    u16* dy_cm = (u16*)work;
+    u16* xs_cm = dy_cm + plane * cout;
+    float* part = (float*)(((uintptr_t)(xs_cm + 3 * plane * ci16) + 255) & ~(uintptr_t)255);
+    hipLaunchKernelGGL(to_cm_kernel, dim3(grid_for(plane * cout)), dim3(256), 0, st,
+                       (const u16*)dy, dcs, dco, cout, cout, dy_cm, nz, ny, nx, nxp, 0);
+    for (int s = 0; s < 3; ++s)
+        hipLaunchKernelGGL(to_cm_kernel, dim3(grid_for(plane * ci16)), dim3(256), 0, st,
+                           (const u16*)x, xcs, xco, cin, ci16, xs_cm + (int64_t)s * ci16 * plane,
+                           nz, ny, nx, nxp, s - 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+#define CASE(A, B) \
+    if (cin == A && cout == B) e = wgrad_t<A, B>(dy_cm, xs_cm, part, nsplit, nz, ny, nxp, st); else
+    OSTR_CNN_WGRAD_SHAPES(CASE)
+#undef CASE
+    return hipErrorInvalidValue;
+    if (e != hipSuccess) return e;
+    const int64_t n = (int64_t)cout * 27 * cin;
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const float*)part,
+                       nsplit, n, dw);
+    if (db)
+        hipLaunchKernelGGL(bias_grad_kernel, dim3(cout), dim3(256), 0, st, (const u16*)dy_cm,
+                           plane, db);
+    return hipGetLastError();
+}
+
+hipError_t launch_cnn_pool(const void* x, int xcs, int xco, void* y, int ycs, int yco, int C,
+                           int nz, int ny, int nx, hipStream_t st) {
+    const int64_t n = (int64_t)(nz / 2) * (ny / 2) * (nx / 2) * C;
+    hipLaunchKernelGGL(pool_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const u16*)x, xcs, xco,
+                       (u16*)y, ycs, yco, C, nz, ny, nx);
+    return hipGetLastError();
+}
+
+hipError_t launch_cnn_pool_bwd(const void* dy, int dcs, int dco, void* dx, int xcs, int xco,
+                               const void* mask, int mcs, int mco, int accumulate, int C, int nz,
+                               int ny, int nx, hipStream_t st) {
+    const int64_t n = (int64_t)nz * ny * nx * C;
+    hipLaunchKernelGGL(pool_bwd_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const u16*)dy, dcs,
+                       dco, (u16*)dx, xcs, xco, (const u16*)mask, mcs, mco, accumulate, C, nz, ny,
+                       nx);
+    return hipGetLastError();
+}
+
+hipError_t launch_cnn_up(const void* x, int xcs, int xco, void* y, int ycs, int yco, int C,
+                         int nz, int ny, int nx, hipStream_t st) {
+    const int64_t n = (int64_t)nz * ny * nx * C;
+    hipLaunchKernelGGL(up_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const u16*)x, xcs, xco,
+                       (u16*)y, ycs, yco, C, nz, ny, nx);
+    return hipGetLastError();
+}
+
+hipError_t launch_cnn_up_bwd(const void* dy, int dcs, int dco, void* dx, int xcs, int xco,
+                             const void* mask, int mcs, int mco, int C, int nz, int ny, int nx,
+                             hipStream_t st) {
+    const int64_t n = (int64_t)(nz / 2) * (ny / 2) * (nx / 2) * C;
+    hipLaunchKernelGGL(up_bwd_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const u16*)dy, dcs,
+                       dco, (u16*)dx, xcs, xco, (const u16*)mask, mcs, mco, C, nz, ny, nx);
+    return hipGetLastError();
+}
+
+hipError_t launch_cnn_input(const float* img, int64_t n, float scale, void* x, hipStream_t st) {
+    hipLaunchKernelGGL(input_kernel, dim3(grid_for(n)), dim3(256), 0, st, img, n, scale, (u16*)x);
+    return hipGetLastError();
+}
+
+hipError_t launch_cnn_output(const void* out, int ocs, const float* img, int64_t n,
+                             float inv_scale, int clamp, float* pred, hipStream_t st) {
+    hipLaunchKernelGGL(output_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const u16*)out, ocs,
+                       img, n, inv_scale, clamp, pred);
+    return hipGetLastError();
+}
+
+int64_t cnn_mse_partials(int64_t n) { return grid_for(n, 256 * 8); }
+
+hipError_t launch_cnn_mse(const float* pred, const float* target, int64_t n, float gscale,
+                          void* dout, int dcs, double* partials, double* result, hipStream_t st) {
+    const unsigned g = (unsigned)cnn_mse_partials(n);
+    hipLaunchKernelGGL(mse_kernel, dim3(g), dim3(256), 0, st, pred, target, n, gscale,
+                       (u16*)dout, dcs, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_reduce_partials(partials, g, result, st);
+}
+
+hipError_t launch_cnn_pack(const float* w, int cin, int cout, void* wf, void* wd,
+                           hipStream_t st) {
+    const int64_t n = (int64_t)cout * kpad(cin) + (int64_t)cin * kpad(cout);
+    hipLaunchKernelGGL(pack_kernel, dim3(grid_for(n)), dim3(256), 0, st, w, cin, cout, (u16*)wf,
+                       (u16*)wd);
+    return hipGetLastError();
+}
+
+int cnn_kpad(int c) { return kpad(c); }
+
+hipError_t launch_cnn_adam(float* p, float* m, float* v, const float* g, int64_t n, float lr,
+                           float b1, float b2, float eps, int step, hipStream_t st) {
+    const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, m, v, g, n, lr, b1,
+                       b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+}  // namespace ostr

@@ -154,6 +154,27 @@ hipError_t launch_div(float* p, float v, int64_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
+// x = max(0, z + c (z - z_prev)): the momentum extrapolation of accelerated
+// SQS / OS-SQS (Nesterov; Kim, Ramani & Fessler 2015), with the non-negativity
+// projection of the OSTR update.
+__global__ __launch_bounds__(256) void extrapolate_kernel(float* __restrict__ x,
+                                                          const float* __restrict__ z,
+                                                          const float* __restrict__ zp, float c,
+                                                          int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = fmaf(c, z[i] - zp[i], z[i]);
+        x[i] = v > 0.0f ? v : 0.0f;
+    }
+}
+
+hipError_t launch_extrapolate(float* x, const float* z, const float* zp, float c, int64_t n,
+                              hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(extrapolate_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, z, zp, c, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(float* p, float v, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, v, n);
@@ -227,6 +248,22 @@ __global__ __launch_bounds__(256) void diff_norms_kernel(const T* __restrict__ a
     }
 }
 
+// partials[k] = sum over block k's strided elements of (x - z)(z - zp), FP64
+// (the gradient-restart test of accelerated SQS; fixed grid: deterministic).
+__global__ __launch_bounds__(256) void restart_dot_kernel(const float* __restrict__ x,
+                                                          const float* __restrict__ z,
+                                                          const float* __restrict__ zp, int64_t n,
+                                                          double* __restrict__ partials) {
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)NORM_BLOCKS * 256) {
+        const double zi = (double)z[i];
+        acc += ((double)x[i] - zi) * (zi - (double)zp[i]);
+    }
+    const double s = block_sum_256(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ p,
                                                               int64_t n,
                                                               double* __restrict__ out) {
@@ -260,6 +297,15 @@ hipError_t launch_diff_norms(const void* a, const void* b, int dtype, int64_t n,
             return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_restart_dot(const float* x, const float* z, const float* zp, int64_t n,
+                              double* partials, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(restart_dot_kernel, dim3(NORM_BLOCKS), dim3(256), 0, st, x, z, zp, n,
+                       partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_reduce_partials(partials, NORM_BLOCKS, out, st);
 }
 
 hipError_t launch_reduce_partials(const double* partials, int64_t n, double* out,

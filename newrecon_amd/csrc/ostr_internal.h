@@ -255,6 +255,40 @@ hipError_t launch_batch_update(const BatchCase* cases, const int* sched, int n_c
                                int rows, int waves, int potential, int prj_dtype,
                                hipStream_t st);
 
+// Warm-start network (cnn.hip). Channels-last BF16 views (pointer, channel
+// stride, channel offset).
+bool cnn_conv_supported(int cin, int cout);
+bool cnn_wgrad_supported(int cin, int cout);
+int cnn_kpad(int c);
+int cnn_nxp(int nx);
+size_t cnn_wgrad_work_bytes(int cin, int cout, int nz, int ny, int nx, int nsplit);
+int64_t cnn_mse_partials(int64_t n);
+hipError_t launch_cnn_conv(int cin, int cout, const void* x, int xcs, int xco, const void* w,
+                           const float* bias, void* y, int ycs, int yco, const void* mask,
+                           int mcs, int mco, int relu, int nz, int ny, int nx, hipStream_t st);
+hipError_t launch_cnn_wgrad(int cin, int cout, const void* x, int xcs, int xco, const void* dy,
+                            int dcs, int dco, int nz, int ny, int nx, float* dw, float* db,
+                            void* work, int nsplit, hipStream_t st);
+hipError_t launch_cnn_pool(const void* x, int xcs, int xco, void* y, int ycs, int yco, int C,
+                           int nz, int ny, int nx, hipStream_t st);
+hipError_t launch_cnn_pool_bwd(const void* dy, int dcs, int dco, void* dx, int xcs, int xco,
+                               const void* mask, int mcs, int mco, int accumulate, int C, int nz,
+                               int ny, int nx, hipStream_t st);
+hipError_t launch_cnn_up(const void* x, int xcs, int xco, void* y, int ycs, int yco, int C,
+                         int nz, int ny, int nx, hipStream_t st);
+hipError_t launch_cnn_up_bwd(const void* dy, int dcs, int dco, void* dx, int xcs, int xco,
+                             const void* mask, int mcs, int mco, int C, int nz, int ny, int nx,
+                             hipStream_t st);
+hipError_t launch_cnn_input(const float* img, int64_t n, float scale, void* x, hipStream_t st);
+hipError_t launch_cnn_output(const void* out, int ocs, const float* img, int64_t n,
+                             float inv_scale, int clamp, float* pred, hipStream_t st);
+hipError_t launch_cnn_mse(const float* pred, const float* target, int64_t n, float gscale,
+                          void* dout, int dcs, double* partials, double* result, hipStream_t st);
+hipError_t launch_cnn_pack(const float* w, int cin, int cout, void* wf, void* wd,
+                           hipStream_t st);
+hipError_t launch_cnn_adam(float* p, float* m, float* v, const float* g, int64_t n, float lr,
+                           float b1, float b2, float eps, int step, hipStream_t st);
+
 // Prior.
 hipError_t launch_prior(const DevGeom& g, float* rdd, float* rd, const void* img,
                         int img_dtype, int potential, float delta, float rho, hipStream_t st);
@@ -279,6 +313,11 @@ hipError_t launch_bp_accum(float* img, const float* num, const float* den, int64
 hipError_t launch_div(float* p, float v, int64_t n, hipStream_t st);
 // Zero/constant fill by a kernel (no hipMemsetAsync, so callers may capture it).
 hipError_t launch_fill(float* p, float v, int64_t n, hipStream_t st);
+hipError_t launch_extrapolate(float* x, const float* z, const float* zp, float c, int64_t n,
+                              hipStream_t st);
+// out[0] = sum (x - z)(z - zp) in FP64; partials: 1024 doubles
+hipError_t launch_restart_dot(const float* x, const float* z, const float* zp, int64_t n,
+                              double* partials, double* out, hipStream_t st);
 
 // FP64 objective pieces. Each writes one double per work-group into
 // `partials` (its count is returned through *n_partials); reduce_partials

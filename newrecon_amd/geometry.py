@@ -127,6 +127,9 @@ class OSTRConfig:
     # and does not converge); "matched": the exact transpose G^T of
     # G = cos(t) A, d = G^T(yry * G 1) (gamma_mode unused): OS-SQS proper.
     backprojector: str = "sbp"
+    # Nesterov momentum on each full OS pass (accelerated OSTR) with the
+    # gradient restart test; off: plain OSTR.
+    momentum: bool = False
 
     @property
     def potential(self) -> int:

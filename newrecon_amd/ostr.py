@@ -85,6 +85,8 @@ def lib():
         "ostr_objective": [vp, vp, i32, vp, vp, vp, i32, ctypes.c_double, ctypes.c_double,
                            ctypes.POINTER(ctypes.c_double), vp],
         "ostr_diff_norms": [vp, vp, vp, i32, i64, ctypes.POINTER(ctypes.c_double), vp],
+        "ostr_extrapolate": [vp, vp, vp, vp, f32, vp],
+        "ostr_restart_dot": [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double), vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -254,6 +256,19 @@ class DBTSystem:
         return (dd / bb) ** 0.5 if bb > 0 else float("inf")
 
     # -- fused --------------------------------------------------------------
+    def extrapolate(self, x, z, z_prev, c):
+        """x = max(0, z + c (z - z_prev)) (momentum step)."""
+        self._check(self._L.ostr_extrapolate(self._h, _p(x), _p(z), _p(z_prev), c,
+                                             self._stream()))
+        return x
+
+    def restart_dot(self, x, z, z_prev) -> float:
+        """sum (x - z)(z - z_prev) in FP64 (synchronous)."""
+        out = ctypes.c_double()
+        self._check(self._L.ostr_restart_dot(self._h, _p(x), _p(z), _p(z_prev),
+                                             ctypes.byref(out), self._stream()))
+        return out.value
+
     def precompute_d(self, y, r=None, gamma_mode="row_sum", d=None):
         d = self.volume() if d is None else d
         self._check(self._L.ostr_precompute_d(self._h, _p(d), _p(y), _p(r),
@@ -397,7 +412,11 @@ class OSTR:
             self.x = self.sys.bp_init(y=self.y_nat, b=self.b_nat).to(img_dtype).contiguous()
         else:
             self.x = dev32(x0).to(img_dtype).contiguous()
+        if self.cfg.momentum and img_dtype != torch.float32:
+            raise ValueError("momentum is implemented for float32 images")
         self.iteration = 0
+        self.z_prev = None  # momentum state (see _momentum_step)
+        self.restarts = 0
         self._precomputed = False
         # optional per-kernel timing: name -> list of (start, end) torch.cuda.Events
         self.events = None
@@ -480,17 +499,52 @@ class OSTR:
             s.update_est(self.x, self.capL, self.RD, self.d, self.RDD, cfg.beta, float(self.M),
                          cfg.guard)
 
+    @property
+    def estimate(self) -> torch.Tensor:
+        """The current image: x, or with momentum the last OS-pass result z
+        (x is then the extrapolated point the next pass starts from)."""
+        return self.z_prev if getattr(self, "z_prev", None) is not None else self.x
+
+    def _momentum_step(self):
+        """After an OS pass from x_k (self.x now holds z_{k+1}):
+        t_{k+1} = (1 + sqrt(1 + 4 t_k^2)) / 2,
+        x_{k+1} = max(0, z_{k+1} + (t_k - 1)/t_{k+1} (z_{k+1} - z_k)),
+        with a reset (t = 1, x = z) when <x_k - z_{k+1}, z_{k+1} - z_k> > 0
+        (gradient restart, O'Donoghue & Candes 2015)."""
+        z = self.x
+        if self.z_prev is None:  # first pass: z_0 = x_0
+            self.z_prev, self._spare, self.t = self._xk.clone(), torch.empty_like(z), 1.0
+        if self.sys.restart_dot(self._xk, z, self.z_prev) > 0.0:
+            self.t = 1.0
+            self.restarts += 1
+            c = 0.0
+        else:
+            tn = 0.5 * (1.0 + (1.0 + 4.0 * self.t * self.t) ** 0.5)
+            c = (self.t - 1.0) / tn
+            self.t = tn
+        xn = self._spare
+        self.sys.extrapolate(xn, z, self.z_prev, c)
+        # rotate: z_prev <- z_{k+1}, x <- x_{k+1}, spare <- z_k
+        self._spare, self.z_prev, self.x = self.z_prev, z, xn
+
     def iterate(self, n_iters: int = 1, callback: Optional[Callable] = None):
-        """Run n_iters full iterations (all subsets); callback(it, x) after each."""
+        """Run n_iters full iterations (all subsets); callback(it, estimate)
+        after each."""
         if not self._precomputed:
             self.precompute()
         for _ in range(n_iters):
+            if self.cfg.momentum:  # x_k, the point this pass starts from
+                if getattr(self, "_xk", None) is None:
+                    self._xk = torch.empty_like(self.x)
+                self._xk.copy_(self.x)
             for m in range(self.M):
                 self._subset(m)
             self.iteration += 1
+            if self.cfg.momentum:
+                self._momentum_step()
             if callback is not None:
-                callback(self.iteration, self.x)
-        return self.x
+                callback(self.iteration, self.estimate)
+        return self.estimate
 
     def run(self):
         return self.iterate(self.cfg.n_iters)
@@ -498,7 +552,7 @@ class OSTR:
     def objective(self, x=None) -> float:
         """FP64 penalized negative log-likelihood of x (default: the current image)."""
         cfg = self.cfg
-        data, pen = self.sys.objective(self.x if x is None else x, self.y_nat, self.b_nat,
+        data, pen = self.sys.objective(self.estimate if x is None else x, self.y_nat, self.b_nat,
                                        self.r_nat, cfg.prior,
                                        cfg.delta if cfg.prior != "q2" else 1.0, self.rho)
         return data + cfg.beta * pen

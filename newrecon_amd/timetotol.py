@@ -1,0 +1,142 @@
+"""Time to a relative RMSE tolerance against the FP64 converged image.
+
+On the 200x100x20 / 9-view workload (seeds 1 / 1001), x* is the FP64
+minimiser of the objective computed by oracle/converged.py (accelerated SQS
+with the exact transpose, FP64 throughout) and committed as
+tests/golden/xstar_small.npz with the SHA-256 of the counts it was computed
+from. Measured here, on the GPU, from data resident in HBM:
+
+* plain: OSTR with the exact transpose, one subset, Nesterov momentum
+  (accelerated SQS), from the one-step backprojected image;
+* warm:  the same iteration from the network's estimate (accepted only if the
+  FP64 objective is lower there than at the backprojection, else the plain
+  start is used: the fallback);
+* the reference's own iteration, plain OSTR with M = 3 subsets, which ends in
+  an ordered-subsets limit cycle and does not reach the tolerance: its floor
+  is reported.
+
+Each path first runs with a distance check after every iteration (FP64 GPU
+reduction) to find the iteration count; the wall-clock is then taken on a
+clean run of that many iterations (start image, precompute, network and
+fallback test included; checks excluded).
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from typing import Optional
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+XSTAR = os.path.join(ROOT, "tests", "golden", "xstar_small.npz")
+WEIGHTS = os.path.join(HERE, "warmstart_small.npz")
+
+
+def load_xstar(path: str = XSTAR):
+    data = np.load(path, allow_pickle=False)
+    return data["xstar"], json.loads(str(data["meta"]))
+
+
+def _data_checksum(y: torch.Tensor) -> str:
+    import hashlib
+    return hashlib.sha256(y.float().cpu().numpy().astype(np.float32).tobytes()).hexdigest()
+
+
+def _rel(sysm, x, xs) -> float:
+    d, n = sysm.diff_norms(x, xs)
+    return float(np.sqrt(d / n))
+
+
+def _sync():
+    torch.cuda.synchronize()
+
+
+def measure(tol: float = 1e-3, max_iters: int = 20000, device=None, weights: str = WEIGHTS,
+            xstar_path: str = XSTAR, ks=(2, 5, 10), m3_iters: int = 3000) -> dict:
+    from .geometry import OSTRConfig, scan_small
+    from .ostr import OSTR
+    from .warmstart import WarmStartNet, make_case, network_image, recon_config
+
+    dev = torch.device(device if device is not None else "cuda:0")
+    scan = scan_small()
+    xs_np, meta = load_xstar(xstar_path)
+    _, y, b = make_case(scan, scan.phantom_seed, dev)
+    if _data_checksum(y) != meta["y_sha256"]:
+        raise RuntimeError("the counts differ from those x* was computed from")
+    xs = torch.from_numpy(xs_np).to(dev)
+    net = WarmStartNet(dev)
+    wmeta = net.load(weights)
+    cfg = recon_config(scan)
+
+    def run(start: str, n_check: Optional[int] = None, n_time: Optional[int] = None):
+        """start 'bp' or 'net'; checks per iteration up to n_check (returns
+        iterations to tol and the error after ks), or times n_time iterations."""
+        _sync()
+        t0 = time.perf_counter()
+        rec = OSTR(scan.geom, scan.angles, y, b, cfg=cfg, x0="bp", device=dev)
+        info = {}
+        if start == "net":
+            bp = rec.x
+            cand = network_image(net, bp)
+            ob, on = rec.objective(bp), rec.objective(cand)
+            info["accepted"] = on < ob
+            info["obj_bp"], info["obj_net"] = ob, on
+            if on < ob:
+                rec.x = cand
+        rec.precompute()
+        if n_time is not None:
+            rec.iterate(n_time)
+            _sync()
+            info["seconds"] = time.perf_counter() - t0
+            return info
+        info["rel_start"] = _rel(rec.sys, rec.x, xs)
+        info["rel_after"] = {}
+        hit = None
+        for k in range(1, n_check + 1):
+            rec.iterate(1)
+            r = _rel(rec.sys, rec.estimate, xs)
+            if k in ks:
+                info["rel_after"][k] = r
+            if r < tol and hit is None:
+                hit = k
+                if k >= max(ks):
+                    break
+        info["iters"] = hit
+        info["rel_end"] = r
+        return info
+
+    out = {"tol": tol, "xstar": {k: meta[k] for k in ("iterations", "fixed_point_residual",
+                                                        "objective") if k in meta},
+           "weights": {k: wmeta.get(k) for k in ("pairs", "target_iters", "epochs")}}
+    for name in ("plain", "warm"):
+        start = "bp" if name == "plain" else "net"
+        chk = run(start, n_check=max_iters)
+        res = dict(chk)
+        if chk["iters"] is not None:
+            run(start, n_time=chk["iters"])  # warm-up of code paths
+            res["seconds"] = run(start, n_time=chk["iters"])["seconds"]
+        else:
+            res["seconds"] = None
+        out[name] = res
+    p, w = out["plain"]["seconds"], out["warm"]["seconds"]
+    out["speedup"] = (p / w) if (p and w) else None
+
+    # the reference iteration (M = 3, plain) and its limit-cycle floor
+    cfg3 = OSTRConfig(n_subsets=3, beta=cfg.beta, delta=cfg.delta, prior=cfg.prior,
+                      backprojector="matched")
+    rec = OSTR(scan.geom, scan.angles, y, b, cfg=cfg3, x0="bp", device=dev)
+    _sync()
+    t0 = time.perf_counter()
+    rec.iterate(m3_iters)
+    _sync()
+    out["ostr_m3"] = {"iters": m3_iters, "seconds": time.perf_counter() - t0,
+                      "rel": _rel(rec.sys, rec.x, xs), "reached": False}
+    return out
+
+
+if __name__ == "__main__":
+    print(json.dumps(measure(), indent=1))

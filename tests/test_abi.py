@@ -1,5 +1,5 @@
 """The C ABI library loads on the CPU host and exports every symbol declared
-in include/ostr_hip.h (no compute calls: there may be no GPU)."""
+in include/*.h (no compute calls: there may be no GPU)."""
 import ctypes
 import os
 import re
@@ -8,21 +8,25 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "ostr_hip.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("ostr_hip.h", "ostr_cnn.h")]
 LIB = os.path.join(ROOT, "newrecon_amd", "libostr_hip.so")
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(ostr_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(ostr_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_the_surface():
     names = declared_functions()
     for required in ("ostr_create", "ostr_destroy", "ostr_last_error", "ostr_fproj",
                      "ostr_gamma", "ostr_bproj_sbp", "ostr_prior", "ostr_update_est",
-                     "ostr_fproj_h", "ostr_update_fused", "ostr_precompute_d"):
+                     "ostr_fproj_h", "ostr_update_fused", "ostr_precompute_d",
+                     "ostr_batch_fproj_h", "ostr_batch_update_fused", "ostr_cnn_conv3d",
+                     "ostr_cnn_wgrad", "ostr_cnn_adam"):
         assert required in names
 
 

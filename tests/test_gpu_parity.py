@@ -239,6 +239,24 @@ def test_ostr_matched_per_iteration_parity(small_case, dev, M, oracle_lib):
         assert all(b2 <= b1 for b1, b2 in zip(objs, objs[1:])), objs
 
 
+def test_accelerated_sqs_matches_fp64_solver(small_case, dev, oracle_lib):
+    """Momentum OSTR (M = 1, exact transpose, gradient restart) on the GPU in
+    FP32 against the FP64 accelerated solver of oracle/converged.py from the
+    same backprojected start: <= 1e-4 rel RMSE after 30 iterations."""
+    from oracle.converged import SQS, solve
+    s, x, y, b = small_case
+    n = 30
+    sqs = SQS(s.geom, s.angles, y, b, beta=1e3, delta=5e-4, nthreads=8)
+    cfg = OSTRConfig(n_subsets=1, beta=1e3, delta=5e-4, prior="hyperbola",
+                     backprojector="matched", momentum=True)
+    rec = OSTR(s.geom, s.angles, y, b, cfg=cfg, x0="bp", device=dev)
+    x0 = rec.x.double().cpu().numpy()
+    ref, info = solve(sqs, x0, n)
+    got = rec.iterate(n).cpu().numpy()
+    assert rec.restarts == info["restarts"]
+    assert rel_rmse(got, ref) <= 1e-4, rel_rmse(got, ref)
+
+
 def test_fused_equals_unfused(small_case, dev):
     s, x, y, b = small_case
     out = []
