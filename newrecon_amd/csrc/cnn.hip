@@ -221,16 +221,20 @@ __global__ void sum_parts_kernel(const float* __restrict__ part, int nsplit, int
     }
 }
 
-// db[co] = sum over voxels of dy (channel-major copy; padding is zero), one
-// work-group per channel, fixed-order reduction.
+// Partial bias gradients: part[b][co] = sum of dy over the b-th of gridDim.y
+// strided shares of channel co's plane (channel-major copy; padding is zero),
+// fixed-order reduction; sum_parts_kernel then adds the shares in order.
+constexpr int BIAS_SPLIT = 64;
 __global__ __launch_bounds__(256) void bias_grad_kernel(const u16* __restrict__ dy_cm,
                                                         int64_t per_channel,
-                                                        float* __restrict__ db) {
+                                                        float* __restrict__ part) {
     const u16* p = dy_cm + (int64_t)blockIdx.x * per_channel;
     double s = 0.0;
-    for (int64_t i = threadIdx.x; i < per_channel; i += 256) s += from_bf16(p[i]);
+    for (int64_t i = (int64_t)blockIdx.y * 256 + threadIdx.x; i < per_channel;
+         i += (int64_t)BIAS_SPLIT * 256)
+        s += from_bf16(p[i]);
     s = block_sum_256(s);
-    if (threadIdx.x == 0) db[blockIdx.x] = (float)s;
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = (float)s;
 }
 
 // ---- pooling / upsampling (channels-last views) ---------------------------
@@ -466,7 +470,7 @@ size_t cnn_wgrad_work_bytes(int cin, int cout, int nz, int ny, int nx, int nspli
     const int64_t plane = (int64_t)nz * ny * cnn_nxp(nx);
     const int64_t ci16 = (cin + 15) / 16 * 16;
     return sizeof(u16) * (size_t)(plane * cout + 3 * plane * ci16) +
-           sizeof(float) * (size_t)nsplit * cout * 27 * cin + 256;
+           sizeof(float) * ((size_t)nsplit * cout * 27 * cin + (size_t)BIAS_SPLIT * cout) + 256;
 }
 
 hipError_t launch_cnn_wgrad(int cin, int cout, const void* x, int xcs, int xco, const void* dy,
@@ -495,9 +499,13 @@ hipError_t launch_cnn_wgrad(int cin, int cout, const void* x, int xcs, int xco, 
     const int64_t n = (int64_t)cout * 27 * cin;
     hipLaunchKernelGGL(sum_parts_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const float*)part,
                        nsplit, n, dw);
-    if (db)
-        hipLaunchKernelGGL(bias_grad_kernel, dim3(cout), dim3(256), 0, st, (const u16*)dy_cm,
-                           plane, db);
+    if (db) {
+        float* bpart = part + (int64_t)nsplit * n;
+        hipLaunchKernelGGL(bias_grad_kernel, dim3(cout, BIAS_SPLIT), dim3(256), 0, st,
+                           (const u16*)dy_cm, plane, bpart);
+        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, st, (const float*)bpart,
+                           BIAS_SPLIT, (int64_t)cout, db);
+    }
     return hipGetLastError();
 }
 
