@@ -227,6 +227,53 @@ int ostr_objective(ostr_handle* h, const void* img, int32_t img_dtype, const flo
 int ostr_diff_norms(ostr_handle* h, const void* a, const void* b, int32_t dtype, int64_t n,
                     double* out, void* stream);
 
+/* ---- OS-SART (same projectors; the reference has the launchers but no driver)
+ * All f32, reference sinogram layout [slot][nbx][nby]. */
+
+/* fprojectCB_1R_GPU_SART / _cos (Tomo_OSTR.cu:2278, :2306): prj = (cos t) A x
+ * for cos_weight != 0, else A x; norm = A 1 (row sums, never cos-weighted);
+ * both 0 for rays that miss the volume. */
+int ostr_fproj_sart(ostr_handle* h, const float* img, float* prj, float* norm, int32_t a0,
+                    int32_t a1, int32_t cos_weight, void* stream);
+
+/* SART_prj_diff (Tomo_OSTR.cu:2537): diff = (p - est) / norm where norm != 0,
+ * else 0, over n bins. */
+int ostr_sart_prj_diff(ostr_handle* h, float* diff, const float* p, const float* est,
+                       const float* norm, int64_t n, void* stream);
+
+/* bprojectCB_4B_GPU_SART (Tomo_OSTR.cu:2460): per voxel, u = SBP(diff) /
+ * SBP-sensitivity over slots [a0,a1) (0 where the sensitivity is 0), then in
+ * place img = max(0, img + lambda u). At most 64 views per call. */
+int ostr_bproj_sart(ostr_handle* h, float* img, const float* diff, int32_t a0, int32_t a1,
+                    float lambda, void* stream);
+
+/* bprojectCB_4B_GPU_SART_R (Tomo_OSTR.cu:2497): as ostr_bproj_sart with
+ * img = max(0, img + lambda (u + beta prior / sensitivity)) (the reference's
+ * sign). */
+int ostr_bproj_sart_r(ostr_handle* h, float* img, const float* diff, const float* prior,
+                      int32_t a0, int32_t a1, float lambda, float beta, void* stream);
+
+/* prior_GPU_SART (Tomo_OSTR.cu:2249): prior += sum over the 27 neighbours of
+ * (1/distance) d / sqrt(1 + (d/delta)^2), d = img[v] - img[neighbour], distance
+ * in voxel-index units (G_Huber_prior :258-325), evaluated in FP32 (the
+ * reference forms each term in double). delta == 0 and prior == img are
+ * errors. */
+int ostr_prior_sart(ostr_handle* h, float* prior, const float* img, float delta, void* stream);
+
+/* Fused OS-SART subset, two launches (no reference counterpart: the reference
+ * runs the five launchers above one after another).
+ * ostr_sart_fproj_diff: diff = SART_prj_diff(p, fproj_sart(img)) over slots
+ * [a0,a1) with p read and diff written detector-row-major [slot][nby][nbx].
+ * ostr_sart_update_fused: x_out = the bprojectCB_4B_GPU_SART[_R] update of x_in
+ * from diff_rows, with the Huber prior of x_in (delta) when beta != 0; x_in and
+ * x_out must differ. At most 64 views per call. */
+int ostr_sart_fproj_diff(ostr_handle* h, const float* img, const float* p_rows,
+                         float* diff_rows, int32_t a0, int32_t a1, int32_t cos_weight,
+                         void* stream);
+int ostr_sart_update_fused(ostr_handle* h, const float* x_in, float* x_out,
+                           const float* diff_rows, int32_t a0, int32_t a1, float lambda,
+                           float beta, float delta, void* stream);
+
 /* x = max(0, z + c (z - z_prev)) over the volume (f32): the momentum step of
  * accelerated OSTR (Nesterov momentum on the OS / SQS update, Kim, Ramani &
  * Fessler, IEEE TMI 2015). No reference counterpart (the reference iterates

@@ -84,10 +84,13 @@ __device__ __forceinline__ float readlane_f(float v, int lane) {
 // row's two tap rows lie on the detector (the common case), the four taps are
 // loaded without masks; otherwise each tap is masked on its own, as the
 // reference does.
-template <typename TP, int ROWS>
+// SENS: also sens[r] = sum over slots of the same bilinear weights applied to
+// the valid taps' indicator (the OS-SART sensitivity, backprj_OSSART_gpu_
+// manyviews, Tomo_OSTR.cu:1856-1888), scaled like the data term.
+template <typename TP, int ROWS, bool SENS = false>
 __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int n,
                                          const VolReader<TP>& P, int ix, int iy0, int iz,
-                                         float total[ROWS]) {
+                                         float total[ROWS], float* sens = nullptr) {
     constexpr int VIEW_CHUNK = 64 / ROWS;
     const uint32_t nbin = (uint32_t)g.nbx * (uint32_t)g.nby;
     const uint32_t nbx = (uint32_t)g.nbx;
@@ -101,7 +104,10 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
     const float sz0 = lane < n ? S.z0[lane] : 0.0f;
     const float snum = lane < n ? S.num[lane] : 0.0f;
 #pragma unroll
-    for (int r = 0; r < ROWS; ++r) total[r] = 0.0f;
+    for (int r = 0; r < ROWS; ++r) {
+        total[r] = 0.0f;
+        if (SENS) sens[r] = 0.0f;
+    }
     for (int c0 = 0; c0 < n; c0 += VIEW_CHUNK) {
         int rt_jl = -2;
         float rt_fj = 0.0f, rt_ey = 0.0f;
@@ -166,6 +172,14 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
                 float u = fmaf(fj, d2, (1.0f - fj) * d1);
                 u = (u * g.dz) * dw;
                 total[r] = total[r] + u;
+                if (SENS) {
+                    // d1_sen = (1-fI)[tap 00] + fI[tap 10], likewise d2_sen
+                    const float d1s = ((i0 && j0) ? omfi : 0.0f) + ((i1 && j0) ? fi : 0.0f);
+                    const float d2s = ((i0 && j1) ? omfi : 0.0f) + ((i1 && j1) ? fi : 0.0f);
+                    float us = fmaf(fj, d2s, (1.0f - fj) * d1s);
+                    us = (us * g.dz) * dw;
+                    sens[r] = sens[r] + us;
+                }
             }
         }
     }
@@ -192,22 +206,53 @@ __global__ __launch_bounds__(256) void sbp_kernel(DevGeom g, SlotTab S, int n_sl
     }
 }
 
-// Prior terms and update of the ROWS voxels (ix, iy0 + r, iz), r = 0..ROWS-1,
-// from a sliding 3x3x3 window: each step loads the 9 values (3 slices x 3
-// columns) of the next row, so a thread reads 9*(ROWS+2) values instead of
-// 26*ROWS. Terms are summed per voxel in the reference's order (dz outer, dy,
-// dx inner), then the voxel is updated and stored at once. CHECKED: the tile
-// touches the volume boundary; a neighbour outside the volume contributes
-// t = 0 (it is replaced by the voxel itself, Tomo_OSTR.cu:238-241).
-template <int POT, bool CHECKED, typename TI, int ROWS>
-__device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI>& X,
-                                            const VolWriter<TI>& XO, const VolReader<float>& D,
-                                            const TileCoord& c, const float total[ROWS],
-                                            const PriorClass& W, float inv_delta, float beta,
-                                            float capl_scale, int guard) {
+// OS-SART backprojection and update (backprj_OSSART_gpu_manyviews and the _R
+// variant with a prior, Tomo_OSTR.cu:1737-2143): per voxel, u = sum / sens
+// over the launch's slots (0 where sens == 0), then, in place,
+//   x = max(0, x + lambda * u)                          (PRIOR false)
+//   x = max(0, x + lambda * (u + beta * prior / sens))  (PRIOR true; the sign is
+//                                                        the reference's)
+template <typename TP, bool PRIOR>
+__global__ __launch_bounds__(256) void sart_update_kernel(DevGeom g, SlotTab S, int n_slots,
+                                                          const TP* __restrict__ diff,
+                                                          float* __restrict__ img,
+                                                          const float* __restrict__ prior,
+                                                          float lambda, float beta) {
+    const TileCoord c = tile_coord<CASE_ROWS, CASE_WAVES>(g, xcd_tile(blockIdx.x, gridDim.x));
+    if (c.iy0 >= g.ny) return;  // see sbp_kernel
+    const VolReader<TP> P(diff, (uint32_t)n_slots * g.nbx * g.nby);
+    float total[CASE_ROWS], sens[CASE_ROWS];
+    sbp_rows<TP, CASE_ROWS, true>(g, S, n_slots, P, c.ix, c.iy0, c.iz, total, sens);
+    if (c.ix >= g.nx) return;
+#pragma unroll
+    for (int r = 0; r < CASE_ROWS; ++r) {
+        const int iy = c.iy0 + r;
+        if (iy >= g.ny) break;
+        const int64_t vi = ((int64_t)c.iz * g.ny + iy) * g.nx + c.ix;
+        float u = 0.0f, bt = 0.0f;
+        if (sens[r] != 0.0f) {
+            u = total[r] / sens[r];
+            if (PRIOR) bt = (beta * prior[vi]) / sens[r];
+        }
+        const float xn = fmaf(lambda, PRIOR ? u + bt : u, img[vi]);
+        img[vi] = xn < 0.0f ? 0.0f : xn;
+    }
+}
+
+// Prior terms of the ROWS voxels (ix, iy0 + r, iz), r = 0..ROWS-1, from a
+// sliding 3x3x3 window: each step loads the 9 values (3 slices x 3 columns) of
+// the next row, so a thread reads 9*(ROWS+2) values instead of 26*ROWS. Terms
+// are summed per voxel in the reference's order (dz outer, dy, dx inner),
+// rdd from 0 and rd from init(r); then epi(r, vi, x_centre, rdd, rd) is called
+// for each voxel inside the volume. CHECKED: the tile touches the volume
+// boundary; a neighbour outside the volume contributes t = 0 (it is replaced
+// by the voxel itself, Tomo_OSTR.cu:238-241).
+template <int POT, bool CHECKED, typename TI, int ROWS, typename Init, typename Epi>
+__device__ __forceinline__ void prior_rows(const DevGeom& g, const VolReader<TI>& X,
+                                           const TileCoord& c, const PriorClass& W,
+                                           float inv_delta, Init init, Epi epi) {
     const int ix = c.ix, iy0 = c.iy0, iz = c.iz;
     const uint32_t nx = g.nx, plane = (uint32_t)g.nx * g.ny;
-    const float two_beta = 2.0f * beta;
     bool cx[3], cz[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -239,7 +284,9 @@ __device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI
     for (int r = 0; r < ROWS; ++r) {
         load_row(2, iy0 + r + 1);
         const float xc = win[1][1][1];
-        float rdd = 0.0f, rd = 0.0f;
+        const int iy = iy0 + r;
+        const bool store = lane_ok && iy < g.ny;
+        float rdd = 0.0f, rd = init(r, store);
 #pragma unroll
         for (int dz = 0; dz < 3; ++dz)
 #pragma unroll
@@ -254,9 +301,34 @@ __device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI
                     rdd = rdd + e;
                     rd = fmaf(e, t, rd);
                 }
-        const int iy = iy0 + r;
-        if (lane_ok && iy < g.ny) {
-            const uint32_t vi = (uint32_t)iz * plane + (uint32_t)iy * nx + (uint32_t)ix;
+        if (store) epi(r, (uint32_t)iz * plane + (uint32_t)iy * nx + (uint32_t)ix, xc, rdd, rd);
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                win[0][dz][dx] = win[1][dz][dx];
+                win[1][dz][dx] = win[2][dz][dx];
+                if (CHECKED) {
+                    ok[0][dz][dx] = ok[1][dz][dx];
+                    ok[1][dz][dx] = ok[2][dz][dx];
+                }
+            }
+    }
+}
+
+// Prior terms and SQS update of the ROWS voxels (the OSTR epilogue of the
+// fused update): capL = M (0 + total), x = max(0, x - (capL + beta rd) /
+// (d + 2 beta rdd)); guard: keep x where the denominator is 0.
+template <int POT, bool CHECKED, typename TI, int ROWS>
+__device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI>& X,
+                                            const VolWriter<TI>& XO, const VolReader<float>& D,
+                                            const TileCoord& c, const float total[ROWS],
+                                            const PriorClass& W, float inv_delta, float beta,
+                                            float capl_scale, int guard) {
+    const float two_beta = 2.0f * beta;
+    prior_rows<POT, CHECKED, TI, ROWS>(
+        g, X, c, W, inv_delta, [](int, bool) { return 0.0f; },
+        [&](int r, uint32_t vi, float xc, float rdd, float rd) {
             // capL: 0 + total, then * M (float), as the reference host does
             const float capl = capl_scale * (0.0f + total[r]);
             const float num = capl + beta * rd;
@@ -269,18 +341,89 @@ __device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI
                 xn = xn < 0.0f ? 0.0f : xn;
             }
             XO(vi, xn);
+        });
+}
+
+// The SART prior G_Huber_prior (Tomo_OSTR.cu:258-325), accumulating:
+//   prior[v] += sum over the 26 neighbours of w d / sqrt(1 + (d/delta)^2),
+// d = x[v] - x[neighbour] (0 for neighbours outside the volume), w = 1/distance
+// in voxel-index units (no z-size ratio). This is the hyperbola gradient term
+// rd of the OSTR prior with rho = 1, evaluated in FP32 (rsq of 1 + q^2, the
+// terms added in the reference's order from prior[v]); the reference forms
+// each term in double and rounds the sum to float after every term.
+__global__ __launch_bounds__(256) void huber_prior_kernel(DevGeom g, PriorClass W,
+                                                          float inv_delta,
+                                                          const float* __restrict__ x,
+                                                          float* __restrict__ prior) {
+    const TileCoord c = tile_coord<CASE_ROWS, CASE_WAVES>(g, xcd_tile(blockIdx.x, gridDim.x));
+    if (c.iy0 >= g.ny) return;  // see sbp_kernel
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const VolReader<float> X(x, nvox);
+    const VolReader<float> PR(prior, nvox);
+    const VolWriter<float> PW(prior, nvox);
+    const uint32_t row0 = ((uint32_t)c.iz * g.ny + (uint32_t)c.iy0) * g.nx + (uint32_t)c.ix;
+    auto init = [&](int r, bool store) { return store ? PR(row0 + (uint32_t)r * g.nx) : 0.0f; };
+    auto epi = [&](int, uint32_t vi, float, float, float rd) { PW(vi, rd); };
+    const int x_first = c.ix - (threadIdx.x & 63);
+    const bool interior = x_first >= 1 && x_first + 64 < g.nx && c.iy0 >= 1 &&
+                          c.iy0 + CASE_ROWS < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
+    if (interior)
+        prior_rows<OSTR_PRIOR_HYPERBOLA, false, float, CASE_ROWS>(g, X, c, W, inv_delta, init,
+                                                                 epi);
+    else
+        prior_rows<OSTR_PRIOR_HYPERBOLA, true, float, CASE_ROWS>(g, X, c, W, inv_delta, init,
+                                                                epi);
+}
+
+// Fused OS-SART subset update (the backprj_OSSART[_R] update with the Huber
+// prior evaluated in the same pass): per voxel, total and sens as
+// sart_update_kernel, prior = 0 + the Huber terms of x_in, then
+// x_out = max(0, x_in + lambda (total/sens [+ beta prior/sens])). Reading
+// x_in and writing x_out keeps the prior's neighbours at their pre-update
+// values, as the separate prior launch does.
+template <bool PRIOR>
+__global__ __launch_bounds__(256) void sart_update_fused_kernel(
+    DevGeom g, SlotTab S, int n_slots, const float* __restrict__ diff,
+    const float* __restrict__ x_in, float* __restrict__ x_out, PriorClass W, float inv_delta,
+    float lambda, float beta) {
+    const TileCoord c = tile_coord<CASE_ROWS, CASE_WAVES>(g, xcd_tile(blockIdx.x, gridDim.x));
+    if (c.iy0 >= g.ny) return;  // see sbp_kernel
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const VolReader<float> P(diff, (uint32_t)n_slots * g.nbx * g.nby);
+    const VolReader<float> X(x_in, nvox);
+    const VolWriter<float> XO(x_out, nvox);
+    float total[CASE_ROWS], sens[CASE_ROWS];
+    sbp_rows<float, CASE_ROWS, true>(g, S, n_slots, P, c.ix, c.iy0, c.iz, total, sens);
+    auto epi = [&](int r, uint32_t vi, float xc, float, float rd) {
+        float u = 0.0f, bt = 0.0f;
+        if (sens[r] != 0.0f) {
+            u = total[r] / sens[r];
+            if (PRIOR) bt = (beta * rd) / sens[r];
         }
+        const float xn = fmaf(lambda, PRIOR ? u + bt : u, xc);
+        XO(vi, xn < 0.0f ? 0.0f : xn);
+    };
+    if (PRIOR) {
+        auto init = [](int, bool) { return 0.0f; };
+        const int x_first = c.ix - (threadIdx.x & 63);
+        const bool interior = x_first >= 1 && x_first + 64 < g.nx && c.iy0 >= 1 &&
+                              c.iy0 + CASE_ROWS < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
+        if (interior)
+            prior_rows<OSTR_PRIOR_HYPERBOLA, false, float, CASE_ROWS>(g, X, c, W, inv_delta,
+                                                                     init, epi);
+        else
+            prior_rows<OSTR_PRIOR_HYPERBOLA, true, float, CASE_ROWS>(g, X, c, W, inv_delta,
+                                                                    init, epi);
+    } else {
+        if (c.ix >= g.nx) return;
+        const uint32_t plane = (uint32_t)g.nx * g.ny;
 #pragma unroll
-        for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                win[0][dz][dx] = win[1][dz][dx];
-                win[1][dz][dx] = win[2][dz][dx];
-                if (CHECKED) {
-                    ok[0][dz][dx] = ok[1][dz][dx];
-                    ok[1][dz][dx] = ok[2][dz][dx];
-                }
-            }
+        for (int r = 0; r < CASE_ROWS; ++r) {
+            const int iy = c.iy0 + r;
+            if (iy >= g.ny) break;
+            const uint32_t vi = (uint32_t)c.iz * plane + (uint32_t)iy * g.nx + (uint32_t)c.ix;
+            epi(r, vi, X(vi), 0.0f, 0.0f);
+        }
     }
 }
 
@@ -513,6 +656,48 @@ hipError_t launch_transpose_bins(const void* src, void* dst, int n_slots, int nb
 
 // `views` is the HOST copy of the slot table (slot parameters go to the kernel
 // by value). prj is detector-row-major [slot][nby][nbx].
+hipError_t launch_sart_update(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                              float* img, const float* diff_rows, const float* prior,
+                              float lambda, float beta, hipStream_t st) {
+    SlotTab S;
+    if (!fill_slots(S, views, a0, a1, g.src_y)) return hipErrorInvalidValue;
+    if (prior)
+        hipLaunchKernelGGL((sart_update_kernel<float, true>), dim3(volume_tiles(g)), dim3(256),
+                           0, st, g, S, a1 - a0, diff_rows, img, prior, lambda, beta);
+    else
+        hipLaunchKernelGGL((sart_update_kernel<float, false>), dim3(volume_tiles(g)),
+                           dim3(256), 0, st, g, S, a1 - a0, diff_rows, img, nullptr, lambda,
+                           0.0f);
+    return hipGetLastError();
+}
+
+hipError_t launch_huber_prior(const DevGeom& g, float* prior, const float* img, float delta,
+                              hipStream_t st) {
+    PriorW W27;
+    prior_weights(W27.w, 1.0f);
+    hipLaunchKernelGGL(huber_prior_kernel, dim3(volume_tiles(g)), dim3(256), 0, st, g,
+                       prior_classes(W27), 1.0f / delta, img, prior);
+    return hipGetLastError();
+}
+
+hipError_t launch_sart_update_fused(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                                    const float* x_in, float* x_out, const float* diff_rows,
+                                    float lambda, float beta, float delta, hipStream_t st) {
+    SlotTab S;
+    if (!fill_slots(S, views, a0, a1, g.src_y)) return hipErrorInvalidValue;
+    PriorW W27;
+    prior_weights(W27.w, 1.0f);
+    const PriorClass W = prior_classes(W27);
+    if (beta != 0.0f)
+        hipLaunchKernelGGL(sart_update_fused_kernel<true>, dim3(volume_tiles(g)), dim3(256), 0,
+                           st, g, S, a1 - a0, diff_rows, x_in, x_out, W, 1.0f / delta, lambda,
+                           beta);
+    else
+        hipLaunchKernelGGL(sart_update_fused_kernel<false>, dim3(volume_tiles(g)), dim3(256), 0,
+                           st, g, S, a1 - a0, diff_rows, x_in, x_out, W, 0.0f, lambda, 0.0f);
+    return hipGetLastError();
+}
+
 hipError_t launch_sbp_rows(const DevGeom& g, const ViewTab* views, int a0, int a1,
                            float* img_accum, const void* prj, int prj_dtype, hipStream_t st) {
     for (int b = a0; b < a1; b += MAX_SLOTS) {

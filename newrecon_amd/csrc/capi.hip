@@ -313,6 +313,115 @@ OSTR_API int ostr_bproj_sbp(ostr_handle* h, float* img_accum, const void* prj,
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_bproj_sbp");
 }
 
+// ---- OS-SART ---------------------------------------------------------------
+
+OSTR_API int ostr_fproj_sart(ostr_handle* h, const float* img, float* prj, float* norm,
+                             int32_t a0, int32_t a1, int32_t cos_weight, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_fproj_sart"))) return rc;
+    if (!img || !prj || !norm) return fail(h, OSTR_E_ARG, "ostr_fproj_sart: NULL buffer");
+    if (prj == norm) return fail(h, OSTR_E_ARG, "ostr_fproj_sart: prj and norm alias");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_sart_project(h->dg, h->views_dev, a0, a1, cos_weight, img, prj,
+                                             norm, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_fproj_sart");
+}
+
+OSTR_API int ostr_sart_prj_diff(ostr_handle* h, float* diff, const float* p, const float* est,
+                                const float* norm, int64_t n, void* stream) {
+    CHECK_HANDLE(h);
+    if (!diff || !p || !est || !norm || n < 0)
+        return fail(h, OSTR_E_ARG, "ostr_sart_prj_diff: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_sart_diff(diff, p, est, norm, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_sart_prj_diff");
+}
+
+namespace {
+int bproj_sart(ostr_handle* h, float* img, const float* diff, const float* prior, int a0,
+               int a1, float lambda, float beta, void* stream, const char* fn) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, fn))) return rc;
+    if (!img || !diff) return fail(h, OSTR_E_ARG, "%s: NULL buffer", fn);
+    if (a1 - a0 > ostr::MAX_SLOTS)
+        return fail(h, OSTR_E_RANGE, "%s: at most %d views per call", fn, ostr::MAX_SLOTS);
+    if ((rc = use_device(h))) return rc;
+    if (a0 == a1) return OSTR_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = (int64_t)(a1 - a0) * h->geom.nbx * h->geom.nby;
+    if ((rc = ensure_work(h, n * sizeof(float), st, fn))) return rc;
+    hipError_t e = ostr::launch_transpose_bins(diff, h->work, a1 - a0, h->geom.nbx, h->geom.nby,
+                                               OSTR_F32, st);
+    if (e == hipSuccess)
+        e = ostr::launch_sart_update(h->dg, h->views_host.data(), a0, a1, img,
+                                     (const float*)h->work, prior, lambda, beta, st);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, fn);
+}
+}  // namespace
+
+OSTR_API int ostr_bproj_sart(ostr_handle* h, float* img, const float* diff, int32_t a0,
+                             int32_t a1, float lambda, void* stream) {
+    return bproj_sart(h, img, diff, nullptr, a0, a1, lambda, 0.0f, stream, "ostr_bproj_sart");
+}
+
+OSTR_API int ostr_bproj_sart_r(ostr_handle* h, float* img, const float* diff, const float* prior,
+                               int32_t a0, int32_t a1, float lambda, float beta, void* stream) {
+    if (h && !prior) return fail(h, OSTR_E_ARG, "ostr_bproj_sart_r: NULL prior");
+    return bproj_sart(h, img, diff, prior, a0, a1, lambda, beta, stream, "ostr_bproj_sart_r");
+}
+
+OSTR_API int ostr_prior_sart(ostr_handle* h, float* prior, const float* img, float delta,
+                             void* stream) {
+    CHECK_HANDLE(h);
+    if (!prior || !img) return fail(h, OSTR_E_ARG, "ostr_prior_sart: NULL buffer");
+    if (prior == img) return fail(h, OSTR_E_ARG, "ostr_prior_sart: prior and img alias");
+    if (delta == 0.0f) return fail(h, OSTR_E_ARG, "ostr_prior_sart: delta cannot be zero");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_huber_prior(h->dg, prior, img, delta, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_prior_sart");
+}
+
+OSTR_API int ostr_sart_fproj_diff(ostr_handle* h, const float* img, const float* p_rows,
+                                  float* diff_rows, int32_t a0, int32_t a1, int32_t cos_weight,
+                                  void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_sart_fproj_diff"))) return rc;
+    if (!img || !p_rows || !diff_rows)
+        return fail(h, OSTR_E_ARG, "ostr_sart_fproj_diff: NULL buffer");
+    if (p_rows == diff_rows) return fail(h, OSTR_E_ARG, "ostr_sart_fproj_diff: p and diff alias");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_sart_project_diff(h->dg, h->views_dev, a0, a1, cos_weight, img,
+                                                  p_rows, diff_rows, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_sart_fproj_diff");
+}
+
+OSTR_API int ostr_sart_update_fused(ostr_handle* h, const float* x_in, float* x_out,
+                                    const float* diff_rows, int32_t a0, int32_t a1,
+                                    float lambda, float beta, float delta, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_sart_update_fused"))) return rc;
+    if (!x_in || !x_out || !diff_rows)
+        return fail(h, OSTR_E_ARG, "ostr_sart_update_fused: NULL buffer");
+    if (x_in == x_out)
+        return fail(h, OSTR_E_ARG, "ostr_sart_update_fused: x_in and x_out must differ");
+    if (beta != 0.0f && delta == 0.0f)
+        return fail(h, OSTR_E_ARG, "ostr_sart_update_fused: delta cannot be zero");
+    if (a1 - a0 > ostr::MAX_SLOTS)
+        return fail(h, OSTR_E_RANGE, "ostr_sart_update_fused: at most %d views per call",
+                    ostr::MAX_SLOTS);
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_sart_update_fused(h->dg, h->views_host.data(), a0, a1, x_in,
+                                                  x_out, diff_rows, lambda, beta, delta,
+                                                  (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_sart_update_fused");
+}
+
 OSTR_API int ostr_bproj_matched(ostr_handle* h, float* img_accum, const void* prj,
                                 int32_t prj_dtype, int32_t a0, int32_t a1, int32_t cos_weight,
                                 int32_t rows_layout, void* stream) {

@@ -168,6 +168,26 @@ __global__ __launch_bounds__(256) void extrapolate_kernel(float* __restrict__ x,
     }
 }
 
+// diff = (p - est) / norm where norm != 0, else 0 (SART_prj_diff_kernel,
+// Tomo_OSTR.cu:2145-2183).
+__global__ __launch_bounds__(256) void sart_diff_kernel(float* __restrict__ diff,
+                                                        const float* __restrict__ p,
+                                                        const float* __restrict__ est,
+                                                        const float* __restrict__ norm,
+                                                        int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        diff[i] = norm[i] != 0.0f ? (p[i] - est[i]) / norm[i] : 0.0f;
+}
+
+hipError_t launch_sart_diff(float* diff, const float* p, const float* est, const float* norm,
+                            int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sart_diff_kernel, dim3(grid_for(n)), dim3(256), 0, st, diff, p, est, norm,
+                       n);
+    return hipGetLastError();
+}
+
 hipError_t launch_extrapolate(float* x, const float* z, const float* zp, float c, int64_t n,
                               hipStream_t st) {
     if (n <= 0) return hipSuccess;

@@ -289,6 +289,28 @@ hipError_t launch_cnn_pack(const float* w, int cin, int cout, void* wf, void* wd
 hipError_t launch_cnn_adam(float* p, float* m, float* v, const float* g, int64_t n, float lr,
                            float b1, float b2, float eps, int step, hipStream_t st);
 
+// OS-SART: forward with row sums (reference layout), projection difference,
+// backprojection + update in place (diff detector-row-major; prior may be
+// NULL), and the Huber-potential prior (accumulating).
+hipError_t launch_sart_project(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                               int cos_weight, const float* img, float* prj, float* norm,
+                               hipStream_t st);
+hipError_t launch_sart_diff(float* diff, const float* p, const float* est, const float* norm,
+                            int64_t n, hipStream_t st);
+hipError_t launch_sart_update(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                              float* img, const float* diff_rows, const float* prior,
+                              float lambda, float beta, hipStream_t st);
+hipError_t launch_huber_prior(const DevGeom& g, float* prior, const float* img, float delta,
+                              hipStream_t st);
+// Fused OS-SART subset: forward + SART_prj_diff (p and diff detector-row-major),
+// then backprojection + Huber prior (beta != 0) + update from x_in into x_out.
+hipError_t launch_sart_project_diff(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                                    int cos_weight, const float* img, const float* p_rows,
+                                    float* diff_rows, hipStream_t st);
+hipError_t launch_sart_update_fused(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                                    const float* x_in, float* x_out, const float* diff_rows,
+                                    float lambda, float beta, float delta, hipStream_t st);
+
 // Prior.
 hipError_t launch_prior(const DevGeom& g, float* rdd, float* rd, const void* img,
                         int img_dtype, int potential, float delta, float rho, hipStream_t st);

@@ -105,11 +105,16 @@ __device__ __forceinline__ float stop_threshold(float lmax) {
 // One Siddon ray from the source S to the bin centre E. Returns the weighted
 // sum (ONES: the plain row sum); hit tells whether the ray meets the volume.
 // COUNT: also counts in-volume segments of positive length into *nseg.
-template <bool ONES, bool COUNT, typename TI>
+// NORM: also accumulates the plain row sum into *norm (sum_norm of the SART
+// walker, Tomo_OSTR.cu:534-539, 601-605: the same segments, added in order,
+// so it equals the ONES walk bit for bit).
+template <bool ONES, bool COUNT, typename TI, bool NORM = false>
 __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img, float sx,
                                       float sy, float sz, float ex, float ey, float ez,
-                                      bool& hit, float& len_out, int& nseg) {
+                                      bool& hit, float& len_out, int& nseg,
+                                      float* norm = nullptr) {
     hit = false;
+    if (NORM) *norm = 0.0f;
     float ddx = ex - sx, ddy = ey - sy, ddz = ez - sz;
     float len = sqrtf(fmaf(ddz, ddz, fmaf(ddy, ddy, ddx * ddx)));
     len_out = len;
@@ -191,6 +196,7 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
             for (int k = 0; k < WALK_BATCH; ++k) {
                 if (ok[k]) {
                     sum = fmaf(v[k], seg[k], sum);
+                    if (NORM) *norm = fmaf(1.0f, seg[k], *norm);
                     if (COUNT && seg[k] > 0.0f) ++nseg;
                 }
             }
@@ -202,6 +208,7 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
             if (inside()) {
                 float v = ONES ? 1.0f : img(vid);
                 sum = fmaf(v, seg, sum);
+                if (NORM) *norm = fmaf(1.0f, seg, *norm);
                 if (COUNT && seg > 0.0f) ++nseg;
             }
             advance();
@@ -211,6 +218,7 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
     if (inside()) {
         float v = ONES ? 1.0f : img(vid);
         sum = fmaf(v, seg, sum);
+        if (NORM) *norm = fmaf(1.0f, seg, *norm);
         if (COUNT && seg > 0.0f) ++nseg;
     }
     hit = true;
@@ -302,6 +310,47 @@ __global__ __launch_bounds__(256) void project_kernel(DevGeom g, const ViewTab* 
     int slot, i, j;
     if (!bin_of_tile<TILE_J>(g, n_slots, xcd_tile(blockIdx.x, gridDim.x), slot, i, j)) return;
     project_bin<KIND, TI, TO>(g, views[a0 + slot], slot, i, j, cos_weight, img, out, y, b, r);
+}
+
+// SART forward projection (ray_trace_gpu_manyangles_direct_notexturememory[_cos],
+// Tomo_OSTR.cu:1175-1311, walker rayTrace3D_GPU_direct_notexturememory
+// :327-611): est = (cos t) sum_j a_ij x_j and norm = sum_j a_ij (no cos), both
+// from one walk and 0 for rays that miss the volume.
+// DIFF = false: est and norm are written in the reference layout [slot][nbx][nby].
+// DIFF = true (the fused OS-SART forward): SART_prj_diff (:2145-2183) is
+// applied at once, diff = (p - est) / norm where norm != 0, else 0, with p read
+// and diff written detector-row-major [slot][nby][nbx], the layout the SART
+// backprojection gathers from; est and norm stay in registers.
+template <bool DIFF>
+__global__ __launch_bounds__(256) void sart_project_kernel(DevGeom g,
+                                                           const ViewTab* __restrict__ views,
+                                                           int a0, int n_slots, int cos_weight,
+                                                           const float* __restrict__ img,
+                                                           float* __restrict__ prj,
+                                                           float* __restrict__ norm,
+                                                           const float* __restrict__ p_rows) {
+    int slot, i, j;
+    if (!bin_of_tile<TILE_J>(g, n_slots, xcd_tile(blockIdx.x, gridDim.x), slot, i, j)) return;
+    const ViewTab v = views[a0 + slot];
+    const float ex = bin_centre(g.x_d0, i, g.dbx);
+    const float ey = bin_centre(g.y_d0, j, g.dby);
+    const float ez = -g.det_r;
+    const VolReader<float> vol(img, (uint32_t)g.nx * g.ny * g.nz);
+    bool hit;
+    float len, nrm;
+    int nseg = 0;
+    const float s = walk<false, false, float, true>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez,
+                                                    hit, len, nseg, &nrm);
+    const float est = hit ? (cos_weight ? s * v.c : s) : 0.0f;
+    nrm = hit ? nrm : 0.0f;
+    if (DIFF) {
+        const int64_t o = ((int64_t)slot * g.nby + j) * g.nbx + i;
+        prj[o] = nrm != 0.0f ? (p_rows[o] - est) / nrm : 0.0f;
+    } else {
+        const int64_t o = ((int64_t)slot * g.nbx + i) * g.nby + j;
+        prj[o] = est;
+        norm[o] = nrm;
+    }
 }
 
 // Batched projection + h (kind 3) over every case of the table in one launch;
@@ -456,6 +505,32 @@ hipError_t launch_batch_project(const BatchCase* cases, const int* sched, int n_
         case OSTR_F16: return batch_fwd_o<_Float16>(rows, cases, sched, n_cases, total, grid, st);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_sart_project(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                               int cos_weight, const float* img, float* prj, float* norm,
+                               hipStream_t st) {
+    const int n_slots = a1 - a0;
+    const int64_t nblocks = (int64_t)((g.nbx + TILE_I - 1) / TILE_I) *
+                            ((g.nby + TILE_J - 1) / TILE_J) * n_slots;
+    if (nblocks == 0) return hipSuccess;
+    if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(sart_project_kernel<false>, dim3((unsigned)nblocks), dim3(256), 0, st, g,
+                       views, a0, n_slots, cos_weight, img, prj, norm, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_sart_project_diff(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                                    int cos_weight, const float* img, const float* p_rows,
+                                    float* diff_rows, hipStream_t st) {
+    const int n_slots = a1 - a0;
+    const int64_t nblocks = (int64_t)((g.nbx + TILE_I - 1) / TILE_I) *
+                            ((g.nby + TILE_J - 1) / TILE_J) * n_slots;
+    if (nblocks == 0) return hipSuccess;
+    if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(sart_project_kernel<true>, dim3((unsigned)nblocks), dim3(256), 0, st, g,
+                       views, a0, n_slots, cos_weight, img, diff_rows, nullptr, p_rows);
+    return hipGetLastError();
 }
 
 hipError_t launch_count_nnz(const DevGeom& g, const ViewTab* views, int a0, int a1,

@@ -87,6 +87,13 @@ def lib():
         "ostr_diff_norms": [vp, vp, vp, i32, i64, ctypes.POINTER(ctypes.c_double), vp],
         "ostr_extrapolate": [vp, vp, vp, vp, f32, vp],
         "ostr_restart_dot": [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double), vp],
+        "ostr_fproj_sart": [vp, vp, vp, vp, i32, i32, i32, vp],
+        "ostr_sart_prj_diff": [vp, vp, vp, vp, vp, i64, vp],
+        "ostr_bproj_sart": [vp, vp, vp, i32, i32, f32, vp],
+        "ostr_bproj_sart_r": [vp, vp, vp, vp, i32, i32, f32, f32, vp],
+        "ostr_prior_sart": [vp, vp, vp, f32, vp],
+        "ostr_sart_fproj_diff": [vp, vp, vp, vp, i32, i32, i32, vp],
+        "ostr_sart_update_fused": [vp, vp, vp, vp, i32, i32, f32, f32, f32, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -363,6 +370,97 @@ class DBTSystem:
         self._check(self._L.ostr_bproj_sbp_rows(self._h, _p(img), _p(prj_rows), _dt(prj_rows),
                                                 a0, a1, self._stream()))
         return img
+
+    # -- OS-SART launchers ----------------------------------------------------
+    def fproj_sart(self, img, a0, a1, prj=None, norm=None, cos_weight=False):
+        """prj = (cos t) A img (cos_weight) or A img, norm = A 1, over [a0, a1)."""
+        prj = self.sinogram(a1 - a0) if prj is None else prj
+        norm = self.sinogram(a1 - a0) if norm is None else norm
+        for t, what in ((img, "SART image"), (prj, "SART projection"), (norm, "SART norm")):
+            _f32(t, what)
+        self._check(self._L.ostr_fproj_sart(self._h, _p(img), _p(prj), _p(norm), a0, a1,
+                                            int(cos_weight), self._stream()))
+        return prj, norm
+
+    def sart_diff(self, p, est, norm, diff=None):
+        """diff = (p - est) / norm where norm != 0, else 0."""
+        diff = torch.empty_like(est) if diff is None else diff
+        for t, what in ((diff, "diff"), (p, "p"), (est, "est"), (norm, "norm")):
+            _f32(t, what)
+            if t.numel() != est.numel():
+                raise ValueError("SART_prj_diff: buffers differ in size")
+        self._check(self._L.ostr_sart_prj_diff(self._h, _p(diff), _p(p), _p(est), _p(norm),
+                                               est.numel(), self._stream()))
+        return diff
+
+    def bproj_sart(self, diff, a0, a1, img, lam, prior=None, beta=0.0):
+        """In place: img = max(0, img + lam (SBP(diff)/sens [+ beta prior/sens]))."""
+        _f32(img, "SART image")
+        _f32(diff, "diff")
+        if prior is None:
+            self._check(self._L.ostr_bproj_sart(self._h, _p(img), _p(diff), a0, a1, lam,
+                                                self._stream()))
+        else:
+            _f32(prior, "prior")
+            self._check(self._L.ostr_bproj_sart_r(self._h, _p(img), _p(diff), _p(prior), a0, a1,
+                                                  lam, beta, self._stream()))
+        return img
+
+    def prior_sart(self, img, delta, prior=None):
+        """prior += sum_n (1/dist_n) d_n / sqrt(1 + (d_n/delta)^2), d_n = img - img_n."""
+        prior = self.volume() if prior is None else prior
+        _f32(img, "SART image")
+        _f32(prior, "prior")
+        self._check(self._L.ostr_prior_sart(self._h, _p(prior), _p(img), delta, self._stream()))
+        return prior
+
+    def sart_fproj_diff(self, img, p_rows, a0, a1, diff_rows=None, cos_weight=False):
+        """Fused forward + SART_prj_diff: diff_rows = (p - A img) / A 1 over
+        [a0, a1), p and diff detector-row-major [S][nby][nbx]."""
+        diff_rows = (torch.empty((a1 - a0, self.geom.nby, self.geom.nbx), device=self.device)
+                     if diff_rows is None else diff_rows)
+        for t, what in ((img, "SART image"), (p_rows, "p"), (diff_rows, "diff")):
+            _f32(t, what)
+        self._check(self._L.ostr_sart_fproj_diff(self._h, _p(img), _p(p_rows), _p(diff_rows),
+                                                 a0, a1, int(cos_weight), self._stream()))
+        return diff_rows
+
+    def sart_update_fused(self, x_in, x_out, diff_rows, a0, a1, lam, beta=0.0, delta=1e-3):
+        """x_out = SART update of x_in from diff_rows (+ Huber prior when beta != 0)."""
+        for t, what in ((x_in, "x_in"), (x_out, "x_out"), (diff_rows, "diff")):
+            _f32(t, what)
+        self._check(self._L.ostr_sart_update_fused(self._h, _p(x_in), _p(x_out), _p(diff_rows),
+                                                   a0, a1, lam, beta, delta, self._stream()))
+        return x_out
+
+    # reference names (Tomo_OSTR.cu:2249-2560)
+    def fprojectCB_1R_GPU_SART(self, estbuf, prj_est, d_normprj, angleStart, angleEnd):
+        """Tomo_OSTR.cu:2278: prj_est = A estbuf, d_normprj = A 1."""
+        return self.fproj_sart(estbuf, angleStart, angleEnd, prj_est, d_normprj, False)
+
+    def fprojectCB_1R_GPU_SART_cos(self, estbuf, prj_est, d_normprj, angleStart, angleEnd):
+        """Tomo_OSTR.cu:2306: prj_est = (cos t) A estbuf, d_normprj = A 1."""
+        return self.fproj_sart(estbuf, angleStart, angleEnd, prj_est, d_normprj, True)
+
+    def SART_prj_diff(self, diff_line, prjbuf, prj_est, normprj, angleStart, angleEnd):
+        """Tomo_OSTR.cu:2537 over the subset's bins [angleEnd - angleStart][nbx][nby]."""
+        n = (angleEnd - angleStart) * self.geom.nbx * self.geom.nby
+        if diff_line.numel() != n:
+            raise ValueError("SART_prj_diff: diff_line does not hold the subset's bins")
+        return self.sart_diff(prjbuf, prj_est, normprj, diff_line)
+
+    def bprojectCB_4B_GPU_SART(self, d_objbuf, d_prjbuf, angleStart, angleEnd, lambda_):
+        """Tomo_OSTR.cu:2460."""
+        return self.bproj_sart(d_prjbuf, angleStart, angleEnd, d_objbuf, lambda_)
+
+    def bprojectCB_4B_GPU_SART_R(self, d_objbuf, d_prjbuf, d_prior, angleStart, angleEnd,
+                                 lambda_, beta):
+        """Tomo_OSTR.cu:2497 (the prior term enters with the reference's + sign)."""
+        return self.bproj_sart(d_prjbuf, angleStart, angleEnd, d_objbuf, lambda_, d_prior, beta)
+
+    def prior_GPU_SART(self, d_prior, d_est, delta):
+        """Tomo_OSTR.cu:2249 (accumulates into d_prior; delta == 0 is an error)."""
+        return self.prior_sart(d_est, delta, d_prior)
 
 
 class OSTR:

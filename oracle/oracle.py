@@ -62,6 +62,18 @@ def lib():
                                  _i, _i, _i, _f32p, _vp]),
             "orc_run_f64": (_i, [_gp, _f32p, _i, _f64p, _f64p, _vp, _vp, _i, _i, _d, _d, _i, _i,
                                  _i, _i, _i, _f64p, _vp]),
+            "orc_sart_prj_diff_f32": (None, [_f32p, _f32p, _f32p, _f32p, _i64]),
+            "orc_sart_prj_diff_f64": (None, [_f64p, _f64p, _f64p, _f64p, _i64]),
+            "orc_bproj_sart_f32": (_i, [_gp, _f32p, _i32p, _f32p, _f32p, _vp, _i, _i, _f, _f,
+                                        _i]),
+            "orc_bproj_sart_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _vp, _i, _i, _d, _d,
+                                        _i]),
+            "orc_prior_huber_f32": (_i, [_gp, _f32p, _f32p, _f, _i]),
+            "orc_prior_huber_f64": (_i, [_gp, _f64p, _f64p, _d, _i]),
+            "orc_run_sart_f32": (_i, [_gp, _f32p, _i, _f32p, _vp, _i, _i, _f, _f, _f, _i, _i,
+                                      _f32p, _vp]),
+            "orc_run_sart_f64": (_i, [_gp, _f32p, _i, _f64p, _vp, _i, _i, _d, _d, _d, _i, _i,
+                                      _f64p, _vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -245,4 +257,54 @@ def run(g: Geometry, angles, y, b, r=None, x0=None, n_subsets=3, n_iters=10, bet
         int(backprojector), nthreads, out.ravel(), _ptr(snaps))
     if rc != 0:
         raise ValueError(f"oracle run: invalid configuration (code {rc})")
+    return out, snaps
+
+
+# ------------------------------------------------------------------ OS-SART --
+def sart_prj_diff(p, est, norm, dtype=np.float32):
+    """(p - est) / norm where norm != 0, else 0 (Tomo_OSTR.cu:2145-2183)."""
+    p, est, norm = _c(p, dtype), _c(est, dtype), _c(norm, dtype)
+    out = np.empty_like(p)
+    getattr(lib(), "orc_sart_prj_diff_" + _dt(dtype))(out.ravel(), p.ravel(), est.ravel(),
+                                                      norm.ravel(), p.size)
+    return out
+
+
+def bproj_sart(g: Geometry, angles, diff, a0, a1, img, lam, index=None, prior=None, beta=0.0,
+               dtype=np.float32, nthreads=0):
+    """SART update of img from diff over slots [a0,a1) (Tomo_OSTR.cu:1737-2143):
+    img = max(0, img + lam (sum/sens [+ beta prior/sens])). Returns the new img."""
+    img = _c(img, dtype).copy()
+    pr = None if prior is None else _c(prior, dtype)
+    getattr(lib(), "orc_bproj_sart_" + _dt(dtype))(
+        ctypes.byref(g.to_c()), _c(angles, np.float32), _idx(index, len(angles)), img.ravel(),
+        _c(diff, dtype).ravel(), _ptr(pr), a0, a1, lam, beta, nthreads)
+    return img
+
+
+def prior_huber(g: Geometry, x, delta, prior=None, dtype=np.float32, nthreads=0):
+    """prior += sum_n (1/dist_n) (x - x_n) / sqrt(1 + ((x - x_n)/delta)^2)
+    (Tomo_OSTR.cu:258-325). Returns the new prior."""
+    prior = np.zeros(g.vol_shape, dtype) if prior is None else _c(prior, dtype).copy()
+    rc = getattr(lib(), "orc_prior_huber_" + _dt(dtype))(
+        ctypes.byref(g.to_c()), prior.ravel(), _c(x, dtype).ravel(), delta, nthreads)
+    if rc != 0:
+        raise ValueError("oracle Huber prior: delta must be non-zero")
+    return prior
+
+
+def run_sart(g: Geometry, angles, p, x0=None, n_subsets=3, n_iters=5, lam=1.0, beta=0.0,
+             delta=1e-3, cos_weight=False, dtype=np.float32, snapshots=False, nthreads=0):
+    """OS-SART loop on the reference's SART launchers; p = line integrals in natural
+    view order. Returns (x, snaps|None)."""
+    nv = len(angles)
+    xi = None if x0 is None else _c(x0, dtype)
+    out = np.empty(g.vol_shape, dtype)
+    snaps = np.empty((n_iters,) + g.vol_shape, dtype) if snapshots else None
+    rc = getattr(lib(), "orc_run_sart_" + _dt(dtype))(
+        ctypes.byref(g.to_c()), _c(angles, np.float32), nv, _c(p, dtype).ravel(), _ptr(xi),
+        n_subsets, n_iters, lam, beta, delta, int(cos_weight), nthreads, out.ravel(),
+        _ptr(snaps))
+    if rc != 0:
+        raise ValueError(f"oracle OS-SART: invalid configuration (code {rc})")
     return out, snaps

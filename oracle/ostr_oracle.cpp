@@ -22,6 +22,10 @@
 //   yry, gamma*yry, regroup, h, update
 //                                    Tomo_OSTR_gold.cpp:183-295
 //   OSTR driver ..................... Tomo_OSTR_gold.cpp:301-918 (Lange), 920-1469 (Q2)
+//   OS-SART: forward + row sum ...... Tomo_OSTR.cu:2278-2333 (kernels :1313-1376 family)
+//            SART_prj_diff .......... Tomo_OSTR.cu:2145-2183
+//            backprojection update .. Tomo_OSTR.cu:1737-2143 (plain and _R)
+//            Huber prior ............ Tomo_OSTR.cu:258-325
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -613,6 +617,156 @@ void regroup(T* dst, const T* src, int n_views, int n_subsets, int64_t nbin) {
             std::memcpy(dst + k, src + (int64_t)(j * n_subsets + m) * nbin, nbin * sizeof(T));
 }
 
+// ------------------------------------------------------------ OS-SART -----
+// SART_prj_diff_kernel (Tomo_OSTR.cu:2145-2183).
+template <typename T>
+void sart_prj_diff(T* diff, const T* p, const T* est, const T* norm, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) diff[i] = norm[i] != 0 ? (p[i] - est[i]) / norm[i] : (T)0;
+}
+
+// backprj_OSSART_gpu_manyviews / _R (Tomo_OSTR.cu:1737-2143): the SBP gather
+// of diff (reference layout) and the sensitivity (the same weights on the
+// valid taps' indicator), then img = max(0, img + lambda (sum/sens [+ beta
+// prior / sens])) in place; sum/sens taken as 0 where sens == 0.
+template <typename T>
+void backproject_sart(const Geo<T>& g, const float* angles, const int* index, T* img,
+                      const T* diff, const T* prior, int a0, int a1, T lambda, T beta,
+                      int nthr) {
+    std::vector<ViewPos<T>> vp(a1 - a0);
+    for (int a = a0; a < a1; ++a) vp[a - a0] = view_pos(g, angles[index[a]]);
+    const int64_t nbin = g.nbin();
+    int rows = g.nz * g.ny;
+#pragma omp parallel for schedule(static) num_threads(nthreads_or_default(nthr))
+    for (int r = 0; r < rows; ++r) {
+        int iz = r / g.ny, iy = r % g.ny;
+        T y1 = cell_centre(g.y_p0, iy, g.dy);
+        T z1 = cell_centre(g.z_p0, iz, g.dz);
+        for (int ix = 0; ix < g.nx; ++ix) {
+            T x1 = cell_centre(g.x_p0, ix, g.dx);
+            T total = 0, sens = 0;
+            for (int a = a0; a < a1; ++a) {
+                const ViewPos<T>& v = vp[a - a0];
+                T x0 = v.x0, y0 = g.src_y, z0 = v.z0;
+                T ex = x0 - x1, ey = y0 - y1, ez = z0 - z1;
+                T dw = fmaT(z0, z0, fmaT(y0, y0, x0 * x0)) / fmaT(ez, ez, fmaT(ey, ey, ex * ex));
+                if (z1 == z0) continue;
+                T t = (-g.det_r - z0) / (z1 - z0);
+                T x2 = fmaT(x1 - x0, t, x0);
+                T y2 = fmaT(y1 - y0, t, y0);
+                T imb = (x2 - g.x_d0) / g.dbx;
+                T jmb = (y2 - g.y_d0) / g.dby;
+                int il = (int)std::floor(imb);
+                if (imb < (T)il + (T)0.5) il -= 1;
+                int jl = (int)std::floor(jmb);
+                if (jmb < (T)jl + (T)0.5) jl -= 1;
+                T fi = imb - ((T)il + (T)0.5);
+                T fj = jmb - ((T)jl + (T)0.5);
+                const T* P = diff + (int64_t)(a - a0) * nbin;
+                bool i0 = il >= 0 && il < g.nbx, i1 = il + 1 >= 0 && il + 1 < g.nbx;
+                bool j0 = jl >= 0 && jl < g.nby, j1 = jl + 1 >= 0 && jl + 1 < g.nby;
+                T d1 = 0, d2 = 0, s1 = 0, s2 = 0;
+                if (i0 && j0) { d1 = ((T)1 - fi) * P[(int64_t)il * g.nby + jl]; s1 = (T)1 - fi; }
+                if (i1 && j0) { d1 = fmaT(fi, P[(int64_t)(il + 1) * g.nby + jl], d1); s1 = s1 + fi; }
+                if (i0 && j1) { d2 = ((T)1 - fi) * P[(int64_t)il * g.nby + jl + 1]; s2 = (T)1 - fi; }
+                if (i1 && j1) { d2 = fmaT(fi, P[(int64_t)(il + 1) * g.nby + jl + 1], d2); s2 = s2 + fi; }
+                T u = fmaT(fj, d2, ((T)1 - fj) * d1);
+                u = (u * g.dz) * dw;
+                T us = fmaT(fj, s2, ((T)1 - fj) * s1);
+                us = (us * g.dz) * dw;
+                total = total + u;
+                sens = sens + us;
+            }
+            int64_t vi = ((int64_t)iz * g.ny + iy) * g.nx + ix;
+            T u = 0, bt = 0;
+            if (sens != 0) {
+                u = total / sens;
+                if (prior) bt = (beta * prior[vi]) / sens;
+            }
+            T xn = fmaT(lambda, prior ? u + bt : u, img[vi]);
+            img[vi] = xn < 0 ? (T)0 : xn;
+        }
+    }
+}
+
+// G_Huber_prior (Tomo_OSTR.cu:258-325), accumulating into prior. The FP32
+// mirror keeps the reference's precision: distance = sqrtf of the integer
+// squared offset (centre 0 -> 1), diff / delta and its square in float, the
+// square root in double rounded to the float denominator, the term in double,
+// the running sum rounded to float after each term.
+template <typename T>
+void prior_huber(const Geo<T>& g, T* prior, const T* x, T delta, int nthr) {
+    int rows = g.nz * g.ny;
+#pragma omp parallel for schedule(static) num_threads(nthreads_or_default(nthr))
+    for (int r = 0; r < rows; ++r) {
+        int iz = r / g.ny, iy = r % g.ny;
+        for (int ix = 0; ix < g.nx; ++ix) {
+            int64_t vi = ((int64_t)iz * g.ny + iy) * g.nx + ix;
+            T xc = x[vi];
+            T acc = prior[vi];
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        float dist = std::sqrt((float)(dx * dx + dy * dy + dz * dz));
+                        if (dist == 0.0f) dist = 1.0f;
+                        int nz = iz + dz, ny = iy + dy, nx = ix + dx;
+                        T xn = xc;
+                        if (nx >= 0 && ny >= 0 && nz >= 0 && nx < g.nx && ny < g.ny && nz < g.nz)
+                            xn = x[((int64_t)nz * g.ny + ny) * g.nx + nx];
+                        T diff = xc - xn;
+                        T q = diff / delta;
+                        T den = (T)std::sqrt(1.0 + (double)(q * q));
+                        acc = (T)((double)acc + ((1.0 / (double)dist) * (double)diff) / (double)den);
+                    }
+            prior[vi] = acc;
+        }
+    }
+}
+
+// OS-SART driver (the reference has the launchers but no driver; this is the
+// standard loop on them): line integrals p (natural view order), the OSTR
+// subset order (slot m*Vs+j = view j*M+m); per subset: SART forward of the
+// subset (cos-weighted when cos_weight), diff, [Huber prior when beta != 0],
+// backprojection update with relaxation lambda.
+template <typename T>
+int run_sart(const ostr_geom& gg, const float* angles, int n_views, const T* p_in,
+             const T* x_init, int M, int n_iters, T lambda, T beta, T delta, int cos_weight,
+             int nthr, T* x_out, T* snapshots) {
+    Geo<T> g(gg);
+    if (M <= 0 || n_views % M != 0) return OSTR_E_ARG;
+    if (M > 0 && n_views / M > 64) return OSTR_E_RANGE;
+    if (beta != 0 && delta == 0) return OSTR_E_ARG;
+    const int vs = n_views / M;
+    const int64_t nbin = g.nbin(), nvox = g.nvox();
+    std::vector<int> idx(n_views);
+    for (int m = 0, k = 0; m < M; ++m)
+        for (int j = 0; j < vs; ++j) idx[k++] = j * M + m;
+    std::vector<T> p((size_t)n_views * nbin);
+    regroup(p.data(), p_in, n_views, M, nbin);
+    std::vector<T> x(nvox), est(vs * nbin), norm(vs * nbin), diff(vs * nbin), pri(nvox);
+    if (x_init)
+        std::memcpy(x.data(), x_init, nvox * sizeof(T));
+    else
+        std::fill(x.begin(), x.end(), (T)0);
+    for (int it = 0; it < n_iters; ++it) {
+        for (int m = 0; m < M; ++m) {
+            int a0 = m * vs, a1 = a0 + vs;
+            project(g, angles, idx.data(), x.data(), est.data(), a0, a1, 0, cos_weight, nthr);
+            project(g, angles, idx.data(), (const T*)nullptr, norm.data(), a0, a1, 2, 0, nthr);
+            sart_prj_diff(diff.data(), p.data() + a0 * nbin, est.data(), norm.data(), vs * nbin);
+            if (beta != 0) {
+                std::fill(pri.begin(), pri.end(), (T)0);
+                prior_huber(g, pri.data(), x.data(), delta, nthr);
+            }
+            backproject_sart(g, angles, idx.data(), x.data(), diff.data(),
+                             beta != 0 ? pri.data() : (const T*)nullptr, a0, a1, lambda, beta,
+                             nthr);
+        }
+        if (snapshots) std::memcpy(snapshots + (int64_t)it * nvox, x.data(), nvox * sizeof(T));
+    }
+    std::memcpy(x_out, x.data(), nvox * sizeof(T));
+    return 0;
+}
+
 // -------------------------------------------------------- objective -----
 template <typename T>
 double objective(const Geo<T>& g, const float* angles, const int* index, int n_views,
@@ -872,6 +1026,54 @@ ORC_EXPORT int orc_run_f32(const ostr_geom* g, const float* ang, int nv, const f
     RunOpts o{n_subsets, n_iters, pot, gamma_mode, guard, nt, beta, delta, backprojector};
     return run<float>(*g, ang, nv, y, b, r, x0, o, x_out, snapshots);
 }
+// OS-SART --------------------------------------------------------------------
+ORC_EXPORT void orc_sart_prj_diff_f32(float* diff, const float* p, const float* est,
+                                      const float* norm, int64_t n) {
+    sart_prj_diff(diff, p, est, norm, n);
+}
+ORC_EXPORT void orc_sart_prj_diff_f64(double* diff, const double* p, const double* est,
+                                      const double* norm, int64_t n) {
+    sart_prj_diff(diff, p, est, norm, n);
+}
+ORC_EXPORT int orc_bproj_sart_f32(const ostr_geom* g, const float* ang, const int* idx,
+                                  float* img, const float* diff, const float* prior, int a0,
+                                  int a1, float lambda, float beta, int nt) {
+    backproject_sart(Geo<float>(*g), ang, idx, img, diff, prior, a0, a1, lambda, beta, nt);
+    return 0;
+}
+ORC_EXPORT int orc_bproj_sart_f64(const ostr_geom* g, const float* ang, const int* idx,
+                                  double* img, const double* diff, const double* prior,
+                                  int a0, int a1, double lambda, double beta, int nt) {
+    backproject_sart(Geo<double>(*g), ang, idx, img, diff, prior, a0, a1, lambda, beta, nt);
+    return 0;
+}
+ORC_EXPORT int orc_prior_huber_f32(const ostr_geom* g, float* prior, const float* x,
+                                   float delta, int nt) {
+    if (delta == 0) return OSTR_E_ARG;
+    prior_huber(Geo<float>(*g), prior, x, delta, nt);
+    return 0;
+}
+ORC_EXPORT int orc_prior_huber_f64(const ostr_geom* g, double* prior, const double* x,
+                                   double delta, int nt) {
+    if (delta == 0) return OSTR_E_ARG;
+    prior_huber(Geo<double>(*g), prior, x, delta, nt);
+    return 0;
+}
+ORC_EXPORT int orc_run_sart_f32(const ostr_geom* g, const float* ang, int nv, const float* p,
+                                const float* x0, int n_subsets, int n_iters, float lambda,
+                                float beta, float delta, int cos_weight, int nt, float* x_out,
+                                float* snapshots) {
+    return run_sart<float>(*g, ang, nv, p, x0, n_subsets, n_iters, lambda, beta, delta,
+                           cos_weight, nt, x_out, snapshots);
+}
+ORC_EXPORT int orc_run_sart_f64(const ostr_geom* g, const float* ang, int nv,
+                                const double* p, const double* x0, int n_subsets, int n_iters,
+                                double lambda, double beta, double delta, int cos_weight,
+                                int nt, double* x_out, double* snapshots) {
+    return run_sart<double>(*g, ang, nv, p, x0, n_subsets, n_iters, lambda, beta, delta,
+                            cos_weight, nt, x_out, snapshots);
+}
+
 ORC_EXPORT int orc_run_f64(const ostr_geom* g, const float* ang, int nv, const double* y,
                            const double* b, const double* r, const double* x0, int n_subsets,
                            int n_iters, double beta, double delta, int pot, int gamma_mode,
