@@ -179,6 +179,18 @@ def bench_batch(args, dev):
     return 0
 
 
+def calibrate_copy(dev):
+    """Three plain device copies of 256 MiB: a kernel of known traffic for
+    scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 SART_BETA, SART_DELTA, SART_LAMBDA = -2e-3, 1e-3, 1.0
 
 
@@ -218,6 +230,8 @@ def bench_sart(args, dev):
     rec.iterate(args.steps)
     torch.cuda.synchronize()
     s_iter = (time.perf_counter() - t0) / args.steps
+    if args.calibrate_copy:
+        calibrate_copy(dev)
 
     # per-launch durations (events on the launch stream) over one more pass
     s, vs = rec.sys, rec.vs
@@ -365,12 +379,7 @@ def main(argv=None):
           flush=True)
 
     if args.calibrate_copy:
-        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibrate_copy(dev)
 
     # per-kernel timing pass (events on the launch stream), same iterations
     rec.events = {}
