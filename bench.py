@@ -256,6 +256,20 @@ def bench_sart(args, dev):
     dominant = max(kms, key=lambda k: kms[k])
     b_dom = sart_bytes(g, vs, dominant)
     achieved = b_dom / (kms[dominant] * 1e-3) / 1e9
+    # the 2x2 supersampled forward (fprojectCB1_4R_GPU_SART) of one subset,
+    # merged in the kernel and with the sub-bin sinogram kept
+    ev4 = {}
+    prj4 = torch.empty((vs, 2 * g.nby, 2 * g.nbx), device=dev)
+    for _ in range(3):
+        for keep in (False, True):
+            a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            s.fproj_4r(rec.x, 0, vs, prj_4r=prj4 if keep else None)
+            z.record()
+            ev4.setdefault(keep, []).append((a, z))
+    torch.cuda.synchronize()
+    k4 = {k: sum(a.elapsed_time(z) for a, z in v) / len(v) for k, v in ev4.items()}
+    del prj4
     # the five reference launchers one after another, for comparison
     ref = OSSART(g, scan.angles, p, x0=rec.x, n_subsets=M, lam=SART_LAMBDA, beta=SART_BETA,
                  delta=SART_DELTA, cos_weight=True, fused=False, device=dev)
@@ -305,6 +319,10 @@ def bench_sart(args, dev):
         "voxel_ray_updates_per_s": round(updates / s_iter, 1),
         "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
         "unfused_s_per_iter": round(s_unfused, 6),
+        "fproj_4r_ms": {"merged": round(k4[False], 4), "with_sub_bins": round(k4[True], 4),
+                        "rays": 4 * vs * g.n_bin,
+                        "algorithmic_bytes": {"merged": g.n_vox * 4 + vs * g.n_bin * 4,
+                                              "with_sub_bins": g.n_vox * 4 + 5 * vs * g.n_bin * 4}},
         "roofline": {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

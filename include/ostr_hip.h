@@ -260,6 +260,16 @@ int ostr_bproj_sart_r(ostr_handle* h, float* img, const float* diff, const float
  * errors. */
 int ostr_prior_sart(ostr_handle* h, float* prior, const float* img, float delta, void* stream);
 
+/* fprojectCB1_4R_GPU_SART (Tomo_OSTR.cu:2398-2433; kernels :1453-1550): the
+ * detector subdivided 2x2, one ray per sub-bin (centre x_d0 + (i + 0.5) Bsize/2),
+ * plain sums (no cos), averaged per bin: prj [slot][nby][nbx] (the merge's
+ * detector-row-major layout) = (s(2i,2j) + s(2i+1,2j) + s(2i,2j+1) +
+ * s(2i+1,2j+1)) / 4; prj_4r [slot][2 nby][2 nbx] receives the sub-bin sums when
+ * not NULL. Missed sub-bins count 0 (the reference leaves them unwritten, and
+ * its merge reads an uninitialised fourth index, :1545-1547). */
+int ostr_fproj_4r(ostr_handle* h, const float* img, float* prj, float* prj_4r, int32_t a0,
+                  int32_t a1, void* stream);
+
 /* Fused OS-SART subset, two launches (no reference counterpart: the reference
  * runs the five launchers above one after another).
  * ostr_sart_fproj_diff: diff = SART_prj_diff(p, fproj_sart(img)) over slots

@@ -385,6 +385,21 @@ OSTR_API int ostr_prior_sart(ostr_handle* h, float* prior, const float* img, flo
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_prior_sart");
 }
 
+OSTR_API int ostr_fproj_4r(ostr_handle* h, const float* img, float* prj, float* prj_4r,
+                           int32_t a0, int32_t a1, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_fproj_4r"))) return rc;
+    if (!img || !prj) return fail(h, OSTR_E_ARG, "ostr_fproj_4r: NULL buffer");
+    if (prj == prj_4r) return fail(h, OSTR_E_ARG, "ostr_fproj_4r: prj and prj_4r alias");
+    if ((int64_t)4 * h->geom.nbx * h->geom.nby >= ((int64_t)1 << 31))
+        return fail(h, OSTR_E_RANGE, "ostr_fproj_4r: too many sub-bins per view");
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_project_4r(h->dg, h->views_dev, a0, a1, img, prj, prj_4r,
+                                           (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_fproj_4r");
+}
+
 OSTR_API int ostr_sart_fproj_diff(ostr_handle* h, const float* img, const float* p_rows,
                                   float* diff_rows, int32_t a0, int32_t a1, int32_t cos_weight,
                                   void* stream) {

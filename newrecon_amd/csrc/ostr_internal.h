@@ -302,6 +302,10 @@ hipError_t launch_sart_update(const DevGeom& g, const ViewTab* views, int a0, in
                               float lambda, float beta, hipStream_t st);
 hipError_t launch_huber_prior(const DevGeom& g, float* prior, const float* img, float delta,
                               hipStream_t st);
+// 2x2 supersampled forward + merge (prj [slot][nby][nbx]; prj4 [slot][2nby][2nbx]
+// or null).
+hipError_t launch_project_4r(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                             const float* img, float* prj, float* prj4, hipStream_t st);
 // Fused OS-SART subset: forward + SART_prj_diff (p and diff detector-row-major),
 // then backprojection + Huber prior (beta != 0) + update from x_in into x_out.
 hipError_t launch_sart_project_diff(const DevGeom& g, const ViewTab* views, int a0, int a1,

@@ -353,6 +353,60 @@ __global__ __launch_bounds__(256) void sart_project_kernel(DevGeom g,
     }
 }
 
+// 2x2 supersampled forward projection with the merge fused
+// (ray_trace_4R_gpu_manyangles_direct_notexturememory + merge_4R_to_1R_global,
+// Tomo_OSTR.cu:1453-1550, launcher fprojectCB1_4R_GPU_SART :2398-2433): one
+// lane per sub-bin of the 2x subdivided detector, sub-bin centre
+// x_d0 + (i + 0.5) Bsize_x / 2, plain sum (no cos); a work-group walks a tile of
+// 64 x 4 sub-bins, and its first wave averages each 2x2 block from LDS in the
+// order s(2i,2j) + s(2i+1,2j) + s(2i,2j+1) + s(2i+1,2j+1), then / 4 (the
+// reference's merge reads an uninitialised fourth index, :1545-1547; this is
+// the intended average). Both outputs are detector-row-major: prj
+// [slot][nby][nbx] and, when prj4 is not null, prj4 [slot][2 nby][2 nbx]. Missed
+// sub-bins are 0 (the reference leaves them unwritten).
+__global__ __launch_bounds__(256) void project_4r_kernel(DevGeom g,
+                                                         const ViewTab* __restrict__ views,
+                                                         int a0, int n_slots,
+                                                         const float* __restrict__ img,
+                                                         float* __restrict__ prj,
+                                                         float* __restrict__ prj4) {
+    __shared__ float sub[4][64];
+    const int nbx2 = 2 * g.nbx, nby2 = 2 * g.nby;
+    const int tiles_j = (nby2 + 3) / 4;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int tj = t % tiles_j;
+    const int rest = t / tiles_j;
+    const int slot = rest % n_slots;
+    const int ti = rest / n_slots;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ii = ti * 64 + lane, jj = tj * 4 + w;
+    float val = 0.0f;
+    if (ii < nbx2 && jj < nby2) {
+        const ViewTab v = views[a0 + slot];
+        const VolReader<float> vol(img, (uint32_t)g.nx * g.ny * g.nz);
+        bool hit;
+        float len;
+        int nseg = 0;
+        const float s = walk<false, false, float>(g, vol, v.x0, g.src_y, v.z0,
+                                                  bin_centre(g.x_d0, ii, 0.5f * g.dbx),
+                                                  bin_centre(g.y_d0, jj, 0.5f * g.dby), -g.det_r,
+                                                  hit, len, nseg);
+        val = hit ? s : 0.0f;
+        if (prj4) prj4[((int64_t)slot * nby2 + jj) * nbx2 + ii] = val;
+    }
+    sub[w][lane] = val;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int r = threadIdx.x >> 5, c = threadIdx.x & 31;
+        const int i = ti * 32 + c, j = tj * 2 + r;
+        if (i < g.nbx && j < g.nby) {
+            const float m = ((sub[2 * r][2 * c] + sub[2 * r][2 * c + 1]) + sub[2 * r + 1][2 * c]) +
+                            sub[2 * r + 1][2 * c + 1];
+            prj[((int64_t)slot * g.nby + j) * g.nbx + i] = m / 4.0f;
+        }
+    }
+}
+
 // Batched projection + h (kind 3) over every case of the table in one launch;
 // schedule as batch_update_kernel.
 template <typename TI, typename TO, int TJ>
@@ -530,6 +584,17 @@ hipError_t launch_sart_project_diff(const DevGeom& g, const ViewTab* views, int 
     if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL(sart_project_kernel<true>, dim3((unsigned)nblocks), dim3(256), 0, st, g,
                        views, a0, n_slots, cos_weight, img, diff_rows, nullptr, p_rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_project_4r(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                             const float* img, float* prj, float* prj4, hipStream_t st) {
+    const int n_slots = a1 - a0;
+    const int64_t nblocks = (int64_t)((2 * g.nbx + 63) / 64) * ((2 * g.nby + 3) / 4) * n_slots;
+    if (nblocks == 0) return hipSuccess;
+    if (nblocks > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(project_4r_kernel, dim3((unsigned)nblocks), dim3(256), 0, st, g, views,
+                       a0, n_slots, img, prj, prj4);
     return hipGetLastError();
 }
 

@@ -93,6 +93,7 @@ def lib():
         "ostr_bproj_sart_r": [vp, vp, vp, vp, i32, i32, f32, f32, vp],
         "ostr_prior_sart": [vp, vp, vp, f32, vp],
         "ostr_sart_fproj_diff": [vp, vp, vp, vp, i32, i32, i32, vp],
+        "ostr_fproj_4r": [vp, vp, vp, vp, i32, i32, vp],
         "ostr_sart_update_fused": [vp, vp, vp, vp, i32, i32, f32, f32, f32, vp],
     }
     for name, args in sig.items():
@@ -432,6 +433,25 @@ class DBTSystem:
         self._check(self._L.ostr_sart_update_fused(self._h, _p(x_in), _p(x_out), _p(diff_rows),
                                                    a0, a1, lam, beta, delta, self._stream()))
         return x_out
+
+    def fproj_4r(self, img, a0, a1, prj=None, prj_4r=None, keep_4r=False):
+        """2x2 supersampled forward: prj [S][nby][nbx] = mean of the 4 sub-bin ray
+        sums (no cos); prj_4r [S][2 nby][2 nbx] filled when given or keep_4r."""
+        g = self.geom
+        prj = torch.empty((a1 - a0, g.nby, g.nbx), device=self.device) if prj is None else prj
+        if prj_4r is None and keep_4r:
+            prj_4r = torch.empty((a1 - a0, 2 * g.nby, 2 * g.nbx), device=self.device)
+        _f32(img, "image")
+        _f32(prj, "projection")
+        if prj_4r is not None:
+            _f32(prj_4r, "sub-bin projection")
+        self._check(self._L.ostr_fproj_4r(self._h, _p(img), _p(prj), _p(prj_4r), a0, a1,
+                                          self._stream()))
+        return (prj, prj_4r) if prj_4r is not None else prj
+
+    def fprojectCB1_4R_GPU_SART(self, d_objbuf, d_prjbuf, d_4R_prjbuf, angleStart, angleEnd):
+        """Tomo_OSTR.cu:2398: d_prjbuf [S][nby][nbx] = merged 2x2 sub-bin sums."""
+        return self.fproj_4r(d_objbuf, angleStart, angleEnd, d_prjbuf, d_4R_prjbuf)
 
     # reference names (Tomo_OSTR.cu:2249-2560)
     def fprojectCB_1R_GPU_SART(self, estbuf, prj_est, d_normprj, angleStart, angleEnd):

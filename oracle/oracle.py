@@ -62,6 +62,8 @@ def lib():
                                  _i, _i, _i, _f32p, _vp]),
             "orc_run_f64": (_i, [_gp, _f32p, _i, _f64p, _f64p, _vp, _vp, _i, _i, _d, _d, _i, _i,
                                  _i, _i, _i, _f64p, _vp]),
+            "orc_fproj_4r_f32": (_i, [_gp, _f32p, _i32p, _f32p, _f32p, _f32p, _i, _i, _i]),
+            "orc_fproj_4r_f64": (_i, [_gp, _f32p, _i32p, _f64p, _f64p, _f64p, _i, _i, _i]),
             "orc_sart_prj_diff_f32": (None, [_f32p, _f32p, _f32p, _f32p, _i64]),
             "orc_sart_prj_diff_f64": (None, [_f64p, _f64p, _f64p, _f64p, _i64]),
             "orc_bproj_sart_f32": (_i, [_gp, _f32p, _i32p, _f32p, _f32p, _vp, _i, _i, _f, _f,
@@ -308,3 +310,15 @@ def run_sart(g: Geometry, angles, p, x0=None, n_subsets=3, n_iters=5, lam=1.0, b
     if rc != 0:
         raise ValueError(f"oracle OS-SART: invalid configuration (code {rc})")
     return out, snaps
+
+
+# ------------------------------------------------------ 2x2 supersampled forward --
+def fproj_4r(g: Geometry, angles, img, a0, a1, index=None, dtype=np.float32, nthreads=0):
+    """Supersampled forward (Tomo_OSTR.cu:1453-1550, 2398-2433). Returns
+    (prj [S][nby][nbx], prj_4r [S][2 nby][2 nbx]), both detector-row-major."""
+    prj = np.empty((a1 - a0, g.nby, g.nbx), dtype)
+    prj4 = np.empty((a1 - a0, 2 * g.nby, 2 * g.nbx), dtype)
+    getattr(lib(), "orc_fproj_4r_" + _dt(dtype))(
+        ctypes.byref(g.to_c()), _c(angles, np.float32), _idx(index, len(angles)),
+        _c(img, dtype).ravel(), prj.ravel(), prj4.ravel(), a0, a1, nthreads)
+    return prj, prj4

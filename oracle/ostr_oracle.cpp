@@ -26,6 +26,7 @@
 //            SART_prj_diff .......... Tomo_OSTR.cu:2145-2183
 //            backprojection update .. Tomo_OSTR.cu:1737-2143 (plain and _R)
 //            Huber prior ............ Tomo_OSTR.cu:258-325
+//   2x2 supersampled forward + merge  Tomo_OSTR.cu:1453-1550, 2398-2433
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -767,6 +768,50 @@ int run_sart(const ostr_geom& gg, const float* angles, int n_views, const T* p_i
     return 0;
 }
 
+// ------------------------------------------------- 2x2 supersampled forward --
+// ray_trace_4R_gpu_manyangles_direct_notexturememory (Tomo_OSTR.cu:1453-1515):
+// one ray per sub-bin of a 2x2 subdivided detector, sub-bin centre
+// x_d0 + (i + 0.5)/2 * Bsize_x (double, rounded), plain sum (no cos), written
+// detector-row-major [slot][2 nby][2 nbx]; the reference leaves missed sub-bins
+// unwritten, here they are 0 (equal when the buffer starts zeroed).
+// merge_4R_to_1R_global (:1517-1550): prj[slot][j][i] = (s(2i,2j) + s(2i+1,2j)
+// + s(2i,2j+1) + s(2i+1,2j+1)) / 4, summed left to right. The reference's merge
+// assigns its third index twice and reads an uninitialised fourth
+// (:1545-1547); the restatement averages the four sub-bins.
+template <typename T>
+void project_4r(const Geo<T>& g, const float* angles, const int* index, const T* img, T* prj,
+                T* prj4, int a0, int a1, int nthr) {
+    const int nbx2 = 2 * g.nbx, nby2 = 2 * g.nby;
+    const T hbx = g.dbx * (T)0.5, hby = g.dby * (T)0.5;
+    int rows = (a1 - a0) * nby2;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads_or_default(nthr))
+    for (int r = 0; r < rows; ++r) {
+        int a = a0 + r / nby2;
+        int jj = r % nby2;
+        ViewPos<T> v = view_pos(g, angles[index[a]]);
+        T ey = cell_centre(g.y_d0, jj, hby);
+        T* out = prj4 + ((int64_t)(a - a0) * nby2 + jj) * nbx2;
+        for (int ii = 0; ii < nbx2; ++ii) {
+            T ex = cell_centre(g.x_d0, ii, hbx);
+            bool hit;
+            T len;
+            T s = walk<T, T>(g, img, v.x0, g.src_y, v.z0, ex, ey, -g.det_r, &hit, &len, nullptr);
+            out[ii] = hit ? s : (T)0;
+        }
+    }
+    const int64_t n_out = (int64_t)(a1 - a0) * g.nby;
+#pragma omp parallel for schedule(static) num_threads(nthreads_or_default(nthr))
+    for (int64_t r = 0; r < n_out; ++r) {
+        int64_t a = r / g.nby;
+        int j = (int)(r % g.nby);
+        const T* s0 = prj4 + (a * nby2 + 2 * j) * nbx2;
+        const T* s1 = s0 + nbx2;
+        T* out = prj + (a * g.nby + j) * g.nbx;
+        for (int i = 0; i < g.nbx; ++i)
+            out[i] = (((s0[2 * i] + s0[2 * i + 1]) + s1[2 * i]) + s1[2 * i + 1]) / (T)4;
+    }
+}
+
 // -------------------------------------------------------- objective -----
 template <typename T>
 double objective(const Geo<T>& g, const float* angles, const int* index, int n_views,
@@ -1026,6 +1071,20 @@ ORC_EXPORT int orc_run_f32(const ostr_geom* g, const float* ang, int nv, const f
     RunOpts o{n_subsets, n_iters, pot, gamma_mode, guard, nt, beta, delta, backprojector};
     return run<float>(*g, ang, nv, y, b, r, x0, o, x_out, snapshots);
 }
+// 2x2 supersampled forward -------------------------------------------------------
+ORC_EXPORT int orc_fproj_4r_f32(const ostr_geom* g, const float* ang, const int* idx,
+                                const float* img, float* prj, float* prj4, int a0, int a1,
+                                int nt) {
+    project_4r(Geo<float>(*g), ang, idx, img, prj, prj4, a0, a1, nt);
+    return 0;
+}
+ORC_EXPORT int orc_fproj_4r_f64(const ostr_geom* g, const float* ang, const int* idx,
+                                const double* img, double* prj, double* prj4, int a0, int a1,
+                                int nt) {
+    project_4r(Geo<double>(*g), ang, idx, img, prj, prj4, a0, a1, nt);
+    return 0;
+}
+
 // OS-SART --------------------------------------------------------------------
 ORC_EXPORT void orc_sart_prj_diff_f32(float* diff, const float* p, const float* est,
                                       const float* norm, int64_t n) {
