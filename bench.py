@@ -333,13 +333,114 @@ def bench_sart(args, dev):
     return 0
 
 
+def bench_recondbt(args, dev):
+    """ReconDBT's SART and ML (SART_dbt.m, ML_dbt.m) on the clinical workload
+    with this build's projector pair (Siddon forward without cos, exact
+    transpose): a step is one SART_dbt iteration (15 views, one update per
+    view); one ML_dbt iteration is timed beside it."""
+    import torch
+
+    from newrecon_amd.geometry import SCANS
+    from newrecon_amd.ostr import DBTSystem
+    from newrecon_amd.recondbt import ml_dbt, sart_dbt
+    from newrecon_amd.workload import make_measurement
+
+    scan = SCANS["clinical"]()
+    g = scan.geom
+    print(f"[bench] workload {scan.name} (ReconDBT SART / ML): phantom and data",
+          file=sys.stderr, flush=True)
+    _, y, b = make_measurement(scan, device=dev)
+    S = DBTSystem(g, scan.angles, device=dev)
+    p = S.line_integrals(y, b)
+    x0 = S.bp_init(y=y, b=b)
+    torch.cuda.synchronize()
+    # the per-view row and column sums are set-up work: time one iteration
+    # with and without them by running 1 and 1 + steps iterations
+    t0 = time.perf_counter()
+    sart_dbt(g, scan.angles, p, x0, 1, 0.15, device=dev)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    sart_dbt(g, scan.angles, p, x0, 1 + args.steps, 0.15, device=dev)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    s_sart = ((t2 - t1) - (t1 - t0)) / args.steps
+    print(f"[bench] SART_dbt {s_sart * 1e3:.1f} ms/iter", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ml_dbt(g, scan.angles, y, x0, b, 1, 2.0, device=dev)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ml_dbt(g, scan.angles, y, x0, b, 3, 2.0, device=dev)
+    torch.cuda.synchronize()
+    s_ml = ((time.perf_counter() - t1) - (t1 - t0)) / 2
+    # the dominant kernel: the exact transpose of one view
+    ev = []
+    one = S.sinogram(1)
+    one.fill_(1.0)
+    img = S.volume()
+    for v in range(3):
+        a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        S.bproj_matched(one, v, v + 1, img=img, cos_weight=False)
+        z.record()
+        ev.append((a, z))
+    torch.cuda.synchronize()
+    k_bt = sum(a.elapsed_time(z) for a, z in ev) / len(ev)
+    b_bt = g.n_bin * 4 + 2 * g.n_vox * 4       # one view read; volume read and written
+    achieved = b_bt / (k_bt * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import numpy as np
+        from oracle import recondbt as R
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(f"[bench] CPU oracle (one view of SART_dbt) on {threads} threads",
+              file=sys.stderr, flush=True)
+        xh = x0.cpu().numpy().astype(np.float64)
+        ph = p[:1].cpu().numpy().astype(np.float64)
+        t0 = time.perf_counter()
+        est = R._G(g, scan.angles, xh, 0)
+        rows = R.O.gamma(g, scan.angles, 0, 1, 1, dtype=np.float64, nthreads=threads)
+        yv = np.where(rows > 5 * g.dz, (ph - est) / np.where(rows > 0, rows, 1.0), 0.0)
+        num = R._Gt(g, scan.angles, yv, 0, threads)
+        den = R._Gt(g, scan.angles, np.ones_like(yv), 0, threads)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            u = np.nan_to_num(num / den)
+        xh += 0.15 * u
+        dt = time.perf_counter() - t0
+        cpu = {"value": round(dt * scan.n_views, 3), "unit": "s/iter", "cores": threads,
+               "kind": "port",
+               "sample": f"1 of {scan.n_views} views of one SART_dbt iteration (forward, row "
+                         f"sums, two FP64 exact transposes), {dt:.2f} s measured, "
+                         f"scaled x{scan.n_views}"}
+
+    out = {
+        "metric": "ReconDBT SART_dbt s/iter (clinical DBT workload)",
+        "value": round(s_sart, 6), "unit": "s/iter",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(s_sart * 1e3, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded breast phantom, Poisson counts)",
+        "config": {"workload": scan.name + "-recondbt", "views": scan.n_views,
+                   "sart_stepsize": 0.15, "ml_stepsize": 2.0, "x0": "BP image"},
+        "ml_dbt_s_per_iter": round(s_ml, 6),
+        "kernel_ms": {"matched_transpose_one_view": round(k_bt, 4)},
+        "roofline": {"kernel": "matched_transpose_one_view", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": b_bt},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="clinical",
-                    choices=["clinical", "small", "batch", "sart"])
+                    choices=["clinical", "small", "batch", "sart", "recondbt"])
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache (JSON) for --config batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -368,6 +469,8 @@ def main(argv=None):
         return bench_batch(args, dev)
     if args.config == "sart":
         return bench_sart(args, dev)
+    if args.config == "recondbt":
+        return bench_recondbt(args, dev)
     scan = SCANS[args.config]()
     g, cfg = scan.geom, scan.recon
     cfg.fused = True

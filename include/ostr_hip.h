@@ -260,6 +260,33 @@ int ostr_bproj_sart_r(ostr_handle* h, float* img, const float* diff, const float
  * errors. */
 int ostr_prior_sart(ostr_handle* h, float* prior, const float* img, float delta, void* stream);
 
+/* ---- ReconDBT SART and ML (ReconDBT/SART_dbt.m, ML_dbt.m) on this build's
+ * projector pair: G = ostr_fproj without cos, G^T = ostr_bproj_matched without
+ * cos. The MATLAB package applies IRT system matrices; these are its per-view
+ * elementwise steps. All f32, n elements, any matching layout. */
+
+/* p = log(b / max(y, 1)) over n elements: the line integrals of counts y
+ * under blank b, as ReconDBT/BP.m:54-60 forms them (no reference launcher). */
+int ostr_line_integrals(ostr_handle* h, float* p, const float* y, const float* b, int64_t n,
+                        void* stream);
+/* SART_dbt.m:55-66: y = (p - est) / l where l > thresh (5 dz in SART_dbt.m),
+ * else y is left unchanged (the MATLAB y persists across views there). */
+int ostr_rdbt_sart_residual(ostr_handle* h, float* y, const float* p, const float* est,
+                            const float* l, float thresh, int64_t n, void* stream);
+/* SART_dbt.m:70-76, 98: x += step * (num / den) with NaN quotients 0; clamp != 0
+ * then sets x = 0 where x < 0 (the end-of-iteration clamp). */
+int ostr_rdbt_sart_update(ostr_handle* h, float* x, const float* num, const float* den,
+                          float step, int32_t clamp, int64_t n, void* stream);
+/* ML_dbt.m:88-96: yb = I0 exp(-drr) (I0 per element, or i0_scalar when i0 is
+ * NULL), ydif = yb - counts, wdrr = drr * yb. */
+int ostr_rdbt_ml_terms(ostr_handle* h, float* ydif, float* wdrr, const float* drr,
+                       const float* counts, const float* i0, float i0_scalar, int64_t n,
+                       void* stream);
+/* ML_dbt.m:100-105: upd = num / den where den != 0 (kept elsewhere), NaN -> 0;
+ * x = x + x * upd * step, NaN -> 0, negative -> 0. */
+int ostr_rdbt_ml_update(ostr_handle* h, float* x, float* upd, const float* num,
+                        const float* den, float step, int64_t n, void* stream);
+
 /* fprojectCB1_4R_GPU_SART (Tomo_OSTR.cu:2398-2433; kernels :1453-1550): the
  * detector subdivided 2x2, one ray per sub-bin (centre x_d0 + (i + 0.5) Bsize/2),
  * plain sums (no cos), averaged per bin: prj [slot][nby][nbx] (the merge's

@@ -385,6 +385,64 @@ OSTR_API int ostr_prior_sart(ostr_handle* h, float* prior, const float* img, flo
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_prior_sart");
 }
 
+OSTR_API int ostr_line_integrals(ostr_handle* h, float* p, const float* y, const float* b,
+                                 int64_t n, void* stream) {
+    CHECK_HANDLE(h);
+    if (!p || !y || !b || n < 0) return fail(h, OSTR_E_ARG, "ostr_line_integrals: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_line_integrals(p, y, b, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_line_integrals");
+}
+
+// ---- ReconDBT SART / ML steps ------------------------------------------------
+
+OSTR_API int ostr_rdbt_sart_residual(ostr_handle* h, float* y, const float* p, const float* est,
+                                     const float* l, float thresh, int64_t n, void* stream) {
+    CHECK_HANDLE(h);
+    if (!y || !p || !est || !l || n < 0)
+        return fail(h, OSTR_E_ARG, "ostr_rdbt_sart_residual: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_rdbt_sart_residual(y, p, est, l, thresh, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_rdbt_sart_residual");
+}
+
+OSTR_API int ostr_rdbt_sart_update(ostr_handle* h, float* x, const float* num, const float* den,
+                                   float step, int32_t clamp, int64_t n, void* stream) {
+    CHECK_HANDLE(h);
+    if (!x || !num || !den || n < 0)
+        return fail(h, OSTR_E_ARG, "ostr_rdbt_sart_update: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_rdbt_sart_update(x, num, den, step, clamp, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_rdbt_sart_update");
+}
+
+OSTR_API int ostr_rdbt_ml_terms(ostr_handle* h, float* ydif, float* wdrr, const float* drr,
+                                const float* counts, const float* i0, float i0_scalar, int64_t n,
+                                void* stream) {
+    CHECK_HANDLE(h);
+    if (!ydif || !wdrr || !drr || !counts || n < 0 || ydif == wdrr)
+        return fail(h, OSTR_E_ARG, "ostr_rdbt_ml_terms: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_rdbt_ml_terms(ydif, wdrr, drr, counts, i0, i0_scalar, n,
+                                              (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_rdbt_ml_terms");
+}
+
+OSTR_API int ostr_rdbt_ml_update(ostr_handle* h, float* x, float* upd, const float* num,
+                                 const float* den, float step, int64_t n, void* stream) {
+    CHECK_HANDLE(h);
+    if (!x || !upd || !num || !den || n < 0)
+        return fail(h, OSTR_E_ARG, "ostr_rdbt_ml_update: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_rdbt_ml_update(x, upd, num, den, step, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_rdbt_ml_update");
+}
+
 OSTR_API int ostr_fproj_4r(ostr_handle* h, const float* img, float* prj, float* prj_4r,
                            int32_t a0, int32_t a1, void* stream) {
     CHECK_HANDLE(h);

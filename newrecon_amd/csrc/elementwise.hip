@@ -180,6 +180,118 @@ __global__ __launch_bounds__(256) void sart_diff_kernel(float* __restrict__ diff
         diff[i] = norm[i] != 0.0f ? (p[i] - est[i]) / norm[i] : 0.0f;
 }
 
+// ReconDBT SART (SART_dbt.m:55-66), one view: y = (p - est) / l where
+// l > thresh (the row sum above 5 dz), else y keeps its value (the MATLAB y
+// persists from the previous view there).
+__global__ __launch_bounds__(256) void rdbt_sart_residual_kernel(
+    float* __restrict__ y, const float* __restrict__ p, const float* __restrict__ est,
+    const float* __restrict__ l, float thresh, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float li = l[i];
+        if (li > thresh) y[i] = (p[i] - est[i]) / li;
+    }
+}
+
+// ReconDBT SART update (SART_dbt.m:70-76, :98): u = num / den with NaN -> 0,
+// x = x + step u; clamp: then x = 0 where x < 0 (after the last view).
+__global__ __launch_bounds__(256) void rdbt_sart_update_kernel(
+    float* __restrict__ x, const float* __restrict__ num, const float* __restrict__ den,
+    float step, int clamp, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float u = num[i] / den[i];
+        if (u != u) u = 0.0f;
+        float xn = x[i] + step * u;
+        if (clamp && xn < 0.0f) xn = 0.0f;
+        x[i] = xn;
+    }
+}
+
+// ReconDBT ML (ML_dbt.m:88-96), one view: yb = I0 exp(-drr), ydif = yb - counts,
+// wdrr = drr yb; I0 per bin (i0 != null) or the scalar i0s.
+__global__ __launch_bounds__(256) void rdbt_ml_terms_kernel(
+    float* __restrict__ ydif, float* __restrict__ wdrr, const float* __restrict__ drr,
+    const float* __restrict__ counts, const float* __restrict__ i0, float i0s, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float l = drr[i];
+        const float yb = (i0 ? i0[i] : i0s) * expf(-l);
+        ydif[i] = yb - counts[i];
+        wdrr[i] = l * yb;
+    }
+}
+
+// ReconDBT ML update (ML_dbt.m:100-105): where den != 0, upd = num / den (upd
+// keeps its previous value elsewhere), NaN -> 0; x = x + (x upd) step, NaN -> 0,
+// then x = 0 where x < 0.
+__global__ __launch_bounds__(256) void rdbt_ml_update_kernel(
+    float* __restrict__ x, float* __restrict__ upd, const float* __restrict__ num,
+    const float* __restrict__ den, float step, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float u = upd[i];
+        if (den[i] != 0.0f) u = num[i] / den[i];
+        if (u != u) u = 0.0f;
+        upd[i] = u;
+        const float xi = x[i];
+        float xn = xi + (xi * u) * step;
+        if (xn != xn) xn = 0.0f;
+        if (xn < 0.0f) xn = 0.0f;
+        x[i] = xn;
+    }
+}
+
+// Line integrals of counts y under blank b: p = log(b / max(y, 1)) (the data
+// step of ReconDBT/BP.m:54-60, as bp_data_kernel forms it).
+__global__ __launch_bounds__(256) void line_integrals_kernel(float* __restrict__ p,
+                                                             const float* __restrict__ y,
+                                                             const float* __restrict__ b,
+                                                             int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = logf(b[i] / fmaxf(y[i], 1.0f));
+}
+
+hipError_t launch_line_integrals(float* p, const float* y, const float* b, int64_t n,
+                                 hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(line_integrals_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, y, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rdbt_sart_residual(float* y, const float* p, const float* est,
+                                     const float* l, float thresh, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rdbt_sart_residual_kernel, dim3(grid_for(n)), dim3(256), 0, st, y, p,
+                       est, l, thresh, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rdbt_sart_update(float* x, const float* num, const float* den, float step,
+                                   int clamp, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rdbt_sart_update_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, num,
+                       den, step, clamp, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rdbt_ml_terms(float* ydif, float* wdrr, const float* drr, const float* counts,
+                                const float* i0, float i0s, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rdbt_ml_terms_kernel, dim3(grid_for(n)), dim3(256), 0, st, ydif, wdrr,
+                       drr, counts, i0, i0s, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rdbt_ml_update(float* x, float* upd, const float* num, const float* den,
+                                 float step, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(rdbt_ml_update_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, upd, num,
+                       den, step, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_sart_diff(float* diff, const float* p, const float* est, const float* norm,
                             int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;

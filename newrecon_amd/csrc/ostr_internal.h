@@ -302,6 +302,17 @@ hipError_t launch_sart_update(const DevGeom& g, const ViewTab* views, int a0, in
                               float lambda, float beta, hipStream_t st);
 hipError_t launch_huber_prior(const DevGeom& g, float* prior, const float* img, float delta,
                               hipStream_t st);
+hipError_t launch_line_integrals(float* p, const float* y, const float* b, int64_t n,
+                                 hipStream_t st);
+// ReconDBT SART / ML (SART_dbt.m, ML_dbt.m) elementwise steps.
+hipError_t launch_rdbt_sart_residual(float* y, const float* p, const float* est,
+                                     const float* l, float thresh, int64_t n, hipStream_t st);
+hipError_t launch_rdbt_sart_update(float* x, const float* num, const float* den, float step,
+                                   int clamp, int64_t n, hipStream_t st);
+hipError_t launch_rdbt_ml_terms(float* ydif, float* wdrr, const float* drr, const float* counts,
+                                const float* i0, float i0s, int64_t n, hipStream_t st);
+hipError_t launch_rdbt_ml_update(float* x, float* upd, const float* num, const float* den,
+                                 float step, int64_t n, hipStream_t st);
 // 2x2 supersampled forward + merge (prj [slot][nby][nbx]; prj4 [slot][2nby][2nbx]
 // or null).
 hipError_t launch_project_4r(const DevGeom& g, const ViewTab* views, int a0, int a1,

@@ -94,6 +94,11 @@ def lib():
         "ostr_prior_sart": [vp, vp, vp, f32, vp],
         "ostr_sart_fproj_diff": [vp, vp, vp, vp, i32, i32, i32, vp],
         "ostr_fproj_4r": [vp, vp, vp, vp, i32, i32, vp],
+        "ostr_line_integrals": [vp, vp, vp, vp, i64, vp],
+        "ostr_rdbt_sart_residual": [vp, vp, vp, vp, vp, f32, i64, vp],
+        "ostr_rdbt_sart_update": [vp, vp, vp, vp, f32, i32, i64, vp],
+        "ostr_rdbt_ml_terms": [vp, vp, vp, vp, vp, vp, f32, i64, vp],
+        "ostr_rdbt_ml_update": [vp, vp, vp, vp, vp, f32, i64, vp],
         "ostr_sart_update_fused": [vp, vp, vp, vp, i32, i32, f32, f32, f32, vp],
     }
     for name, args in sig.items():
@@ -453,6 +458,59 @@ class DBTSystem:
         """Tomo_OSTR.cu:2398: d_prjbuf [S][nby][nbx] = merged 2x2 sub-bin sums."""
         return self.fproj_4r(d_objbuf, angleStart, angleEnd, d_prjbuf, d_4R_prjbuf)
 
+    def line_integrals(self, y, b, p=None):
+        """p = log(b / max(y, 1))."""
+        p = torch.empty_like(y) if p is None else p
+        for t in (p, y, b):
+            _f32(t, "line-integral operand")
+            if t.numel() != y.numel():
+                raise ValueError("line_integrals: buffers differ in size")
+        self._check(self._L.ostr_line_integrals(self._h, _p(p), _p(y), _p(b), y.numel(),
+                                                self._stream()))
+        return p
+
+    # -- ReconDBT SART / ML steps (ReconDBT/SART_dbt.m, ML_dbt.m) -----------------
+    def rdbt_sart_residual(self, y, p, est, l, thresh):
+        """y = (p - est) / l where l > thresh; elsewhere y is kept."""
+        for t in (y, p, est, l):
+            _f32(t, "SART_dbt operand")
+            if t.numel() != y.numel():
+                raise ValueError("SART_dbt residual: buffers differ in size")
+        self._check(self._L.ostr_rdbt_sart_residual(self._h, _p(y), _p(p), _p(est), _p(l),
+                                                    thresh, y.numel(), self._stream()))
+        return y
+
+    def rdbt_sart_update(self, x, num, den, step, clamp=False):
+        """x += step * num / den (NaN -> 0); clamp: then negatives -> 0."""
+        for t in (x, num, den):
+            _f32(t, "SART_dbt operand")
+            if t.numel() != x.numel():
+                raise ValueError("SART_dbt update: buffers differ in size")
+        self._check(self._L.ostr_rdbt_sart_update(self._h, _p(x), _p(num), _p(den), step,
+                                                  int(clamp), x.numel(), self._stream()))
+        return x
+
+    def rdbt_ml_terms(self, ydif, wdrr, drr, counts, i0=None, i0_scalar=0.0):
+        """yb = I0 exp(-drr): ydif = yb - counts, wdrr = drr * yb."""
+        for t in (ydif, wdrr, drr, counts) + ((i0,) if i0 is not None else ()):
+            _f32(t, "ML_dbt operand")
+            if t.numel() != drr.numel():
+                raise ValueError("ML_dbt terms: buffers differ in size")
+        self._check(self._L.ostr_rdbt_ml_terms(self._h, _p(ydif), _p(wdrr), _p(drr),
+                                               _p(counts), _p(i0), i0_scalar, drr.numel(),
+                                               self._stream()))
+        return ydif, wdrr
+
+    def rdbt_ml_update(self, x, upd, num, den, step):
+        """upd = num / den where den != 0; x = max(0, x + x upd step), NaN -> 0."""
+        for t in (x, upd, num, den):
+            _f32(t, "ML_dbt operand")
+            if t.numel() != x.numel():
+                raise ValueError("ML_dbt update: buffers differ in size")
+        self._check(self._L.ostr_rdbt_ml_update(self._h, _p(x), _p(upd), _p(num), _p(den),
+                                                step, x.numel(), self._stream()))
+        return x
+
     # reference names (Tomo_OSTR.cu:2249-2560)
     def fprojectCB_1R_GPU_SART(self, estbuf, prj_est, d_normprj, angleStart, angleEnd):
         """Tomo_OSTR.cu:2278: prj_est = A estbuf, d_normprj = A 1."""
@@ -528,6 +586,17 @@ class OSTR:
             self.x = self.sys.volume(img_dtype)
         elif isinstance(x0, str) and x0 == "bp":
             self.x = self.sys.bp_init(y=self.y_nat, b=self.b_nat).to(img_dtype).contiguous()
+        elif isinstance(x0, str) and x0 in ("sart", "ml"):
+            # ReconDBT's iterative initialisers from the BP image, with the
+            # example's settings (dbt_recon_circular_example.m:92, :100)
+            from .recondbt import ml_dbt, sart_dbt
+            xbp = self.sys.bp_init(y=self.y_nat, b=self.b_nat)
+            if x0 == "sart":
+                p = self.sys.line_integrals(self.y_nat, self.b_nat)
+                xi = sart_dbt(geom, angles, p, xbp, 2, 0.15, device=self.dev)
+            else:
+                xi = ml_dbt(geom, angles, self.y_nat, xbp, self.b_nat, 3, 2.0, device=self.dev)
+            self.x = xi.to(img_dtype).contiguous()
         else:
             self.x = dev32(x0).to(img_dtype).contiguous()
         if self.cfg.momentum and img_dtype != torch.float32:
