@@ -311,6 +311,17 @@ int ostr_sart_update_fused(ostr_handle* h, const float* x_in, float* x_out,
                            const float* diff_rows, int32_t a0, int32_t a1, float lambda,
                            float beta, float delta, void* stream);
 
+/* The momentum step of accelerated OSTR with the restart decision on the
+ * device, so an iteration needs no host round trip and can be captured in a
+ * HIP graph (no reference counterpart). state: 4 doubles in device memory,
+ * {t, c, restarts, last test value}, set to {1, 0, 0, 0} by ostr_momentum_init.
+ * ostr_momentum_step: dot = <x_k - z, z - z_prev> (FP64, fixed order); dot > 0
+ * restarts (t = 1, c = 0, restarts += 1), else t' = (1 + sqrt(1 + 4 t^2)) / 2,
+ * c = (t - 1) / t'; then x_next = max(0, z + c (z - z_prev)). Asynchronous. */
+int ostr_momentum_init(ostr_handle* h, double* state, void* stream);
+int ostr_momentum_step(ostr_handle* h, float* x_next, const float* z, const float* z_prev,
+                       const float* x_k, double* state, void* stream);
+
 /* x = max(0, z + c (z - z_prev)) over the volume (f32): the momentum step of
  * accelerated OSTR (Nesterov momentum on the OS / SQS update, Kim, Ramani &
  * Fessler, IEEE TMI 2015). No reference counterpart (the reference iterates

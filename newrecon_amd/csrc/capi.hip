@@ -672,6 +672,32 @@ OSTR_API int ostr_restart_dot(ostr_handle* h, const float* x, const float* z,
     return OSTR_OK;
 }
 
+OSTR_API int ostr_momentum_init(ostr_handle* h, double* state, void* stream) {
+    CHECK_HANDLE(h);
+    if (!state) return fail(h, OSTR_E_ARG, "ostr_momentum_init: NULL state");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_momentum_init(state, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_momentum_init");
+}
+
+OSTR_API int ostr_momentum_step(ostr_handle* h, float* x_next, const float* z,
+                                const float* z_prev, const float* x_k, double* state,
+                                void* stream) {
+    CHECK_HANDLE(h);
+    if (!x_next || !z || !z_prev || !x_k || !state)
+        return fail(h, OSTR_E_ARG, "ostr_momentum_step: NULL buffer");
+    if (x_next == z || x_next == z_prev || x_next == x_k)
+        return fail(h, OSTR_E_ARG, "ostr_momentum_step: x_next aliases an input");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = ensure_dwork(h, 1024 + 1, st, "ostr_momentum_step"))) return rc;
+    int64_t n = (int64_t)h->geom.nx * h->geom.ny * h->geom.nz;
+    hipError_t e = ostr::launch_momentum_step(x_next, z, z_prev, x_k, n, h->dwork, state, st);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_momentum_step");
+}
+
 OSTR_API int ostr_regroup(ostr_handle* h, float* dst, const float* src, int32_t n_subsets,
                           int32_t to_rows, void* stream) {
     CHECK_HANDLE(h);

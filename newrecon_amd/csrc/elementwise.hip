@@ -405,7 +405,77 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __re
     if (threadIdx.x == 0) out[0] = s;
 }
 
+// Momentum state (4 doubles): t, c, restarts, last restart-test value.
+__global__ void momentum_init_kernel(double* __restrict__ state) {
+    if (threadIdx.x == 0) {
+        state[0] = 1.0;
+        state[1] = 0.0;
+        state[2] = 0.0;
+        state[3] = 0.0;
+    }
+}
+
+// One block: the restart test dot = sum of the partials (the same fixed order
+// as reduce_partials_kernel), then, on the device, the Nesterov coefficient:
+// dot > 0 -> restart (t = 1, c = 0); else t' = (1 + sqrt(1 + 4 t^2)) / 2,
+// c = (t - 1) / t', t = t' (O'Donoghue & Candes 2015 gradient restart).
+__global__ __launch_bounds__(256) void momentum_coef_kernel(const double* __restrict__ p,
+                                                            int64_t n,
+                                                            double* __restrict__ state) {
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) acc += p[i];
+    const double dot = block_sum_256(acc);
+    if (threadIdx.x == 0) {
+        const double t = state[0];
+        if (dot > 0.0) {
+            state[0] = 1.0;
+            state[1] = 0.0;
+            state[2] = state[2] + 1.0;
+        } else {
+            const double tn = 0.5 * (1.0 + sqrt(1.0 + 4.0 * t * t));
+            state[1] = (t - 1.0) / tn;
+            state[0] = tn;
+        }
+        state[3] = dot;
+    }
+}
+
+// extrapolate_kernel with c = (float) state[1] read on the device.
+__global__ __launch_bounds__(256) void extrapolate_state_kernel(float* __restrict__ x,
+                                                                const float* __restrict__ z,
+                                                                const float* __restrict__ zp,
+                                                                const double* __restrict__ state,
+                                                                int64_t n) {
+    const float c = (float)state[1];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = fmaf(c, z[i] - zp[i], z[i]);
+        x[i] = v > 0.0f ? v : 0.0f;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_momentum_init(double* state, hipStream_t st) {
+    hipLaunchKernelGGL(momentum_init_kernel, dim3(1), dim3(64), 0, st, state);
+    return hipGetLastError();
+}
+
+hipError_t launch_momentum_step(float* x_next, const float* z, const float* z_prev,
+                                const float* x_k, int64_t n, double* partials, double* state,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(restart_dot_kernel, dim3(NORM_BLOCKS), dim3(256), 0, st, x_k, z, z_prev,
+                       n, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(momentum_coef_kernel, dim3(1), dim3(256), 0, st, partials,
+                       (int64_t)NORM_BLOCKS, state);
+    e = hipGetLastError();
+    if (e != hipSuccess || n <= 0) return e;
+    hipLaunchKernelGGL(extrapolate_state_kernel, dim3(grid_for(n)), dim3(256), 0, st, x_next, z,
+                       z_prev, state, n);
+    return hipGetLastError();
+}
 
 int64_t diff_norms_partials(int64_t) { return 2 * NORM_BLOCKS; }
 

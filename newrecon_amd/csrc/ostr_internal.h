@@ -302,6 +302,12 @@ hipError_t launch_sart_update(const DevGeom& g, const ViewTab* views, int a0, in
                               float lambda, float beta, hipStream_t st);
 hipError_t launch_huber_prior(const DevGeom& g, float* prior, const float* img, float delta,
                               hipStream_t st);
+// Accelerated SQS momentum with the restart decision on the device (no host
+// round trip): state = {t, c, restarts, last test value}.
+hipError_t launch_momentum_init(double* state, hipStream_t st);
+hipError_t launch_momentum_step(float* x_next, const float* z, const float* z_prev,
+                                const float* x_k, int64_t n, double* partials, double* state,
+                                hipStream_t st);
 hipError_t launch_line_integrals(float* p, const float* y, const float* b, int64_t n,
                                  hipStream_t st);
 // ReconDBT SART / ML (SART_dbt.m, ML_dbt.m) elementwise steps.

@@ -95,6 +95,8 @@ def lib():
         "ostr_sart_fproj_diff": [vp, vp, vp, vp, i32, i32, i32, vp],
         "ostr_fproj_4r": [vp, vp, vp, vp, i32, i32, vp],
         "ostr_line_integrals": [vp, vp, vp, vp, i64, vp],
+        "ostr_momentum_init": [vp, vp, vp],
+        "ostr_momentum_step": [vp, vp, vp, vp, vp, vp, vp],
         "ostr_rdbt_sart_residual": [vp, vp, vp, vp, vp, f32, i64, vp],
         "ostr_rdbt_sart_update": [vp, vp, vp, vp, f32, i32, i64, vp],
         "ostr_rdbt_ml_terms": [vp, vp, vp, vp, vp, vp, f32, i64, vp],
@@ -274,6 +276,18 @@ class DBTSystem:
         self._check(self._L.ostr_extrapolate(self._h, _p(x), _p(z), _p(z_prev), c,
                                              self._stream()))
         return x
+
+    def momentum_init(self, state):
+        """state (4 doubles on the device) = {t, c, restarts, test} = {1, 0, 0, 0}."""
+        self._check(self._L.ostr_momentum_init(self._h, _p(state), self._stream()))
+        return state
+
+    def momentum_step(self, x_next, z, z_prev, x_k, state):
+        """Restart test, Nesterov coefficient and extrapolation, all on the
+        device (asynchronous): x_next = max(0, z + c (z - z_prev))."""
+        self._check(self._L.ostr_momentum_step(self._h, _p(x_next), _p(z), _p(z_prev), _p(x_k),
+                                               _p(state), self._stream()))
+        return x_next
 
     def restart_dot(self, x, z, z_prev) -> float:
         """sum (x - z)(z - z_prev) in FP64 (synchronous)."""
@@ -603,7 +617,8 @@ class OSTR:
             raise ValueError("momentum is implemented for float32 images")
         self.iteration = 0
         self.z_prev = None  # momentum state (see _momentum_step)
-        self.restarts = 0
+        self._mstate = None
+        self._graph = None
         self._precomputed = False
         # optional per-kernel timing: name -> list of (start, end) torch.cuda.Events
         self.events = None
@@ -692,43 +707,88 @@ class OSTR:
         (x is then the extrapolated point the next pass starts from)."""
         return self.z_prev if getattr(self, "z_prev", None) is not None else self.x
 
+    @property
+    def restarts(self) -> int:
+        """Momentum restarts so far (reads the device state)."""
+        return 0 if self._mstate is None else int(self._mstate[2].item())
+
     def _momentum_step(self):
         """After an OS pass from x_k (self.x now holds z_{k+1}):
         t_{k+1} = (1 + sqrt(1 + 4 t_k^2)) / 2,
         x_{k+1} = max(0, z_{k+1} + (t_k - 1)/t_{k+1} (z_{k+1} - z_k)),
         with a reset (t = 1, x = z) when <x_k - z_{k+1}, z_{k+1} - z_k> > 0
-        (gradient restart, O'Donoghue & Candes 2015)."""
+        (gradient restart, O'Donoghue & Candes 2015). The test and t live on
+        the device: no host round trip."""
         z = self.x
         if self.z_prev is None:  # first pass: z_0 = x_0
-            self.z_prev, self._spare, self.t = self._xk.clone(), torch.empty_like(z), 1.0
-        if self.sys.restart_dot(self._xk, z, self.z_prev) > 0.0:
-            self.t = 1.0
-            self.restarts += 1
-            c = 0.0
-        else:
-            tn = 0.5 * (1.0 + (1.0 + 4.0 * self.t * self.t) ** 0.5)
-            c = (self.t - 1.0) / tn
-            self.t = tn
+            self.z_prev, self._spare = self._xk.clone(), torch.empty_like(z)
+            self._mstate = torch.empty(4, dtype=torch.float64, device=self.dev)
+            self.sys.momentum_init(self._mstate)
         xn = self._spare
-        self.sys.extrapolate(xn, z, self.z_prev, c)
+        self.sys.momentum_step(xn, z, self.z_prev, self._xk, self._mstate)
         # rotate: z_prev <- z_{k+1}, x <- x_{k+1}, spare <- z_k
         self._spare, self.z_prev, self.x = self.z_prev, z, xn
 
-    def iterate(self, n_iters: int = 1, callback: Optional[Callable] = None):
+    def _one_iteration(self):
+        if self.cfg.momentum:  # x_k, the point this pass starts from
+            if getattr(self, "_xk", None) is None:
+                self._xk = torch.empty_like(self.x)
+            self._xk.copy_(self.x)
+        for m in range(self.M):
+            self._subset(m)
+        self.iteration += 1
+        if self.cfg.momentum:
+            self._momentum_step()
+
+    def _buffers(self):
+        return tuple(t.data_ptr() if torch.is_tensor(t) else None
+                     for t in (self.x, getattr(self, "x_next", None), self.z_prev,
+                               getattr(self, "_spare", None)))
+
+    def _capture(self, max_iters: int = 12):
+        """Record iterations into a HIP graph until the image buffers are back
+        where they started (the momentum rotation has period 3, the fused
+        double buffer 2 per subset); returns the number recorded."""
+        start, it0 = self._buffers(), self.iteration
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for k in range(1, max_iters + 1):
+                self._one_iteration()
+                if self._buffers() == start:
+                    break
+            else:
+                raise RuntimeError("image buffers do not return to their start")
+        self.iteration = it0
+        self._graph, self._graph_iters, self._graph_start = g, k, start
+        return k
+
+    def iterate(self, n_iters: int = 1, callback: Optional[Callable] = None,
+                graph: bool = False):
         """Run n_iters full iterations (all subsets); callback(it, estimate)
-        after each."""
+        after each. graph=True replays the iterations from a HIP graph recorded
+        on first use (one eager iteration first, so every work buffer exists;
+        no callback)."""
         if not self._precomputed:
             self.precompute()
+        if graph and callback is None and n_iters > 0:
+            if self._graph is None:
+                self._one_iteration()
+                n_iters -= 1
+                torch.cuda.synchronize(self.dev)
+                self._capture()
+            # the graph starts from the buffer assignment it was recorded with:
+            # eager iterations first until the rotation is back there
+            while n_iters > 0 and self._buffers() != self._graph_start:
+                self._one_iteration()
+                n_iters -= 1
+            k = self._graph_iters
+            if self._buffers() == self._graph_start:
+                for _ in range(n_iters // k):
+                    self._graph.replay()
+                    self.iteration += k
+                n_iters %= k
         for _ in range(n_iters):
-            if self.cfg.momentum:  # x_k, the point this pass starts from
-                if getattr(self, "_xk", None) is None:
-                    self._xk = torch.empty_like(self.x)
-                self._xk.copy_(self.x)
-            for m in range(self.M):
-                self._subset(m)
-            self.iteration += 1
-            if self.cfg.momentum:
-                self._momentum_step()
+            self._one_iteration()
             if callback is not None:
                 callback(self.iteration, self.estimate)
         return self.estimate

@@ -18,7 +18,9 @@ from. Measured here, on the GPU, from data resident in HBM:
 Each path first runs with a distance check after every iteration (FP64 GPU
 reduction) to find the iteration count; the wall-clock is then taken on a
 clean run of that many iterations (start image, precompute, network and
-fallback test included; checks excluded).
+fallback test included; checks excluded), the iterations replayed from a HIP
+graph (the momentum restart decision stays on the device, so an iteration
+needs no host round trip; the graph run equals the eager one bit for bit).
 """
 from __future__ import annotations
 
@@ -89,7 +91,7 @@ def measure(tol: float = 1e-3, max_iters: int = 20000, device=None, weights: str
                 rec.x = cand
         rec.precompute()
         if n_time is not None:
-            rec.iterate(n_time)
+            rec.iterate(n_time, graph=True)
             _sync()
             info["seconds"] = time.perf_counter() - t0
             return info
