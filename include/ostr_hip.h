@@ -202,6 +202,17 @@ int ostr_update_fused(ostr_handle* h, const void* x_in, void* x_out, int32_t img
                       int32_t a1, float beta, float capl_scale, int32_t potential,
                       float delta, float z_xy_ratio, int32_t guard, void* stream);
 
+/* ostr_update_fused with the exact transpose in place of SBP (no reference
+ * counterpart): x_out = x_in - (capl_scale (0 + G^T h) + beta RD) /
+ * (d + 2 beta RDD), G^T the transpose of ostr_fproj (cos-weighted, as
+ * ostr_bproj_matched), the prior of x_in, guard as ostr_update_est. FP32 image
+ * and h; h detector-row-major; at most 64 views; x_in and x_out must differ.
+ * Equals ostr_bproj_matched + ostr_prior + ostr_update_est bit for bit. */
+int ostr_update_fused_matched(ostr_handle* h, const float* x_in, float* x_out, const float* d,
+                              const float* hsub, int32_t a0, int32_t a1, float beta,
+                              float capl_scale, int32_t potential, float delta, float z_xy_ratio,
+                              int32_t guard, void* stream);
+
 /* ------------------------------------------------------- measurement -- */
 
 /* Number of (ray, voxel) pairs with a_ij > 0 for slots [a0,a1): the exact nnz

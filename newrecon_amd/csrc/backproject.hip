@@ -27,6 +27,7 @@
 //   x_out = max(0, x - (capL + b RD)/(d + 2 b RDD))   (update_est :265-295)
 // reading x_in and writing x_out (double-buffered, so the prior sees the
 // pre-update image as the reference's sequential order does).
+#include "matched_device.h"
 #include "prior_device.h"
 
 namespace ostr {
@@ -456,6 +457,44 @@ __device__ __forceinline__ void update_tile(const DevGeom& g, const SlotTab& S, 
                                       guard);
 }
 
+// Fused OSTR update with the exact transpose (the matched counterpart of
+// update_fused_kernel): per voxel capL = M (0 + sum_views cos sum_i a_ij h_i)
+// from matched_voxel, then the prior window on x_in and the SQS update into
+// x_out, the same epilogue (update_rows). h detector-row-major. One row per
+// lane (MATCHED_ROWS = 1): the gather dominates and is latency-bound, so the
+// launch keeps as many waves as the standalone transpose.
+constexpr int MATCHED_ROWS = 1;
+
+template <int POT>
+__global__ __launch_bounds__(256) void update_fused_matched_kernel(
+    DevGeom g, MatchedSlots S, int n_slots, const float* __restrict__ hsub,
+    const float* __restrict__ x_in, float* __restrict__ x_out, const float* __restrict__ d,
+    PriorClass W, float inv_delta, float beta, float capl_scale, int guard) {
+    constexpr int R = MATCHED_ROWS;
+    const TileCoord c = tile_coord<R, CASE_WAVES>(g, xcd_tile(blockIdx.x, gridDim.x));
+    if (c.iy0 >= g.ny) return;  // see sbp_kernel
+    const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
+    const VolReader<float> P(hsub, (uint32_t)n_slots * g.nbx * g.nby);
+    const VolReader<float> X(x_in, nvox);
+    const VolWriter<float> XO(x_out, nvox);
+    const VolReader<float> D(d, nvox);
+    float total[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        total[r] = (c.ix < g.nx && c.iy0 + r < g.ny)
+                       ? matched_voxel<float>(g, S, n_slots, 1, P, c.ix, c.iy0 + r, c.iz)
+                       : 0.0f;
+    const int x_first = c.ix - (threadIdx.x & 63);
+    const bool interior = x_first >= 1 && x_first + 64 < g.nx && c.iy0 >= 1 &&
+                          c.iy0 + R < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
+    if (interior)
+        update_rows<POT, false, float, R>(g, X, XO, D, c, total, W, inv_delta, beta, capl_scale,
+                                          guard);
+    else
+        update_rows<POT, true, float, R>(g, X, XO, D, c, total, W, inv_delta, beta, capl_scale,
+                                         guard);
+}
+
 template <int POT, typename TI, typename TP>
 __global__ __launch_bounds__(256) void update_fused_kernel(
     DevGeom g, SlotTab S, int n_slots, const TP* __restrict__ hsub, const TI* __restrict__ x_in,
@@ -695,6 +734,46 @@ hipError_t launch_sart_update_fused(const DevGeom& g, const ViewTab* views, int 
     else
         hipLaunchKernelGGL(sart_update_fused_kernel<false>, dim3(volume_tiles(g)), dim3(256), 0,
                            st, g, S, a1 - a0, diff_rows, x_in, x_out, W, 0.0f, lambda, 0.0f);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_fused_matched(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                                       const float* x_in, float* x_out, const float* d,
+                                       const float* hsub, float beta, float capl_scale,
+                                       int potential, float delta, float rho, int guard,
+                                       hipStream_t st) {
+    if (a1 - a0 > MAX_SLOTS || a1 < a0) return hipErrorInvalidValue;
+    MatchedSlots S;
+    for (int a = a0; a < a1; ++a) {
+        S.x0[a - a0] = views[a].x0;
+        S.z0[a - a0] = views[a].z0;
+        S.c[a - a0] = views[a].c;
+    }
+    PriorW W27;
+    prior_weights(W27.w, rho);
+    const PriorClass W = prior_classes(W27);
+    const float inv_delta = potential == OSTR_PRIOR_Q2 ? 0.0f : 1.0f / delta;
+    const dim3 grid(volume_tiles(g, MATCHED_ROWS * CASE_WAVES));
+    const int n = a1 - a0;
+    switch (potential) {
+        case OSTR_PRIOR_Q2:
+            hipLaunchKernelGGL(update_fused_matched_kernel<OSTR_PRIOR_Q2>, grid, dim3(256), 0, st,
+                               g, S, n, hsub, x_in, x_out, d, W, inv_delta, beta, capl_scale,
+                               guard);
+            break;
+        case OSTR_PRIOR_LANGE:
+            hipLaunchKernelGGL(update_fused_matched_kernel<OSTR_PRIOR_LANGE>, grid, dim3(256), 0,
+                               st, g, S, n, hsub, x_in, x_out, d, W, inv_delta, beta, capl_scale,
+                               guard);
+            break;
+        case OSTR_PRIOR_HYPERBOLA:
+            hipLaunchKernelGGL(update_fused_matched_kernel<OSTR_PRIOR_HYPERBOLA>, grid, dim3(256),
+                               0, st, g, S, n, hsub, x_in, x_out, d, W, inv_delta, beta,
+                               capl_scale, guard);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -806,6 +806,35 @@ OSTR_API int ostr_update_fused(ostr_handle* h, const void* x_in, void* x_out, in
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_update_fused");
 }
 
+OSTR_API int ostr_update_fused_matched(ostr_handle* h, const float* x_in, float* x_out,
+                                       const float* d, const float* hsub, int32_t a0,
+                                       int32_t a1, float beta, float capl_scale,
+                                       int32_t potential, float delta, float z_xy_ratio,
+                                       int32_t guard, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_update_fused_matched"))) return rc;
+    if (!x_in || !x_out || !d || !hsub)
+        return fail(h, OSTR_E_ARG, "ostr_update_fused_matched: NULL buffer");
+    if (x_in == x_out)
+        return fail(h, OSTR_E_ARG, "ostr_update_fused_matched: x_in and x_out alias");
+    if (potential < OSTR_PRIOR_Q2 || potential > OSTR_PRIOR_HYPERBOLA)
+        return fail(h, OSTR_E_ARG, "ostr_update_fused_matched: unknown potential %d", potential);
+    if (potential != OSTR_PRIOR_Q2 && delta == 0.0f)
+        return fail(h, OSTR_E_ARG, "ostr_update_fused_matched: delta cannot be zero");
+    if (!(z_xy_ratio > 0.0f))
+        return fail(h, OSTR_E_ARG, "ostr_update_fused_matched: z_xy_ratio must be > 0");
+    if (a1 - a0 > ostr::MAX_SLOTS)
+        return fail(h, OSTR_E_RANGE, "ostr_update_fused_matched: at most %d views per subset",
+                    ostr::MAX_SLOTS);
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_update_fused_matched(h->dg, h->views_host.data(), a0, a1, x_in,
+                                                     x_out, d, hsub, beta, capl_scale, potential,
+                                                     delta, z_xy_ratio, guard,
+                                                     (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_update_fused_matched");
+}
+
 OSTR_API int ostr_count_nnz(ostr_handle* h, int32_t a0, int32_t a1, int64_t* nnz_out,
                             void* stream) {
     CHECK_HANDLE(h);

@@ -1,52 +1,19 @@
 // matched.hip - exact transpose of the Siddon forward projector, voxel-driven.
 //
 // No reference counterpart: the reference backprojects with the unmatched
-// bilinear SBP (Tomo_OSTR.cu:1553-1734). This kernel computes, for every voxel
-// j, x_j += sum_views cos(t) sum_i a_ij p_i with a_ij the length of ray i
-// (source -> bin centre, the forward projector's ray) inside the voxel box,
-// which is the entry a_ij of the forward walk (Tomo_OSTR.cu:612-890) in exact
-// arithmetic. The candidate rays of a voxel are the bins whose centres fall
-// in the bounding box of the voxel's 8 corners projected from the source onto
-// the detector plane; each candidate is clipped against the voxel box (slab
-// method). Deterministic: every voxel is owned by one lane, no atomics.
-//
-// Mapping: 64 lanes on consecutive voxels along x, one row per wave (4 rows
-// per work-group), one slice per grid z; the sinogram is detector-row-major
-// [slot][nby][nbx], so neighbouring lanes read neighbouring bins.
-//
-// Precision: each ray is parametrised from its detector end, p(t) = E + t (S -
-// E). The volume lies next to the detector, so the clip parameters are small
-// (t ~ 0.03 instead of ~0.97 from the source) and their FP32 resolution, which
-// sets the error of a segment length, is ~16x finer.
-#include "ostr_internal.h"
+// bilinear SBP (Tomo_OSTR.cu:1553-1734). This kernel accumulates, for every
+// voxel j, x_j += sum_views cos(t) sum_i a_ij p_i (matched_voxel in
+// matched_device.h: the candidate bins of the voxel's shadow, each ray clipped
+// against the voxel box).
+#include "matched_device.h"
 
 namespace ostr {
 
 namespace {
 
-struct MatchedSlots {
-    float x0[MAX_SLOTS], z0[MAX_SLOTS], c[MAX_SLOTS];
-};
-
-__device__ __forceinline__ float bin_centre(float origin, int i, float size) {
-    return (float)((double)origin + ((double)i + 0.5) * (double)size);
-}
-
-// Parametric overlap of the line s + t (e - s) with [lo, hi] along one axis.
-__device__ __forceinline__ void slab(float s, float e, float lo, float hi, float& t0,
-                                     float& t1) {
-    const float d = e - s;
-    if (d != 0.0f) {
-        const float inv = 1.0f / d;
-        float a = (lo - s) * inv, b = (hi - s) * inv;
-        t0 = fmaxf(t0, fminf(a, b));
-        t1 = fminf(t1, fmaxf(a, b));
-    } else if (!(s > lo && s < hi)) {
-        t1 = -1.0f;  // parallel and outside: empty
-        t0 = 1.0f;
-    }
-}
-
+// One voxel per lane: 64 lanes on consecutive voxels along x, one row per
+// wave (4 rows per work-group), one slice per grid z. Deterministic: every
+// voxel is owned by one lane, no atomics.
 template <typename TP>
 __global__ __launch_bounds__(256) void matched_kernel(DevGeom g, MatchedSlots S, int n_slots,
                                                       int cos_weight,
@@ -56,59 +23,8 @@ __global__ __launch_bounds__(256) void matched_kernel(DevGeom g, MatchedSlots S,
     const int iy = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int iz = blockIdx.z;
     if (ix >= g.nx || iy >= g.ny) return;
-    const float bx0 = fmaf((float)ix, g.dx, g.x_p0), bx1 = fmaf((float)(ix + 1), g.dx, g.x_p0);
-    const float by0 = fmaf((float)iy, g.dy, g.y_p0), by1 = fmaf((float)(iy + 1), g.dy, g.y_p0);
-    const float bz0 = fmaf((float)iz, g.dz, g.z_p0), bz1 = fmaf((float)(iz + 1), g.dz, g.z_p0);
-    const float sy = g.src_y, ez = -g.det_r;
-    const uint32_t nbin = (uint32_t)g.nbx * g.nby;
-    const VolReader<TP> P(prj, (uint32_t)n_slots * nbin);
-    float total = 0.0f;
-    for (int a = 0; a < n_slots; ++a) {
-        const float sx = S.x0[a], sz = S.z0[a];
-        // shadow of the voxel on the detector plane (corners at the two z faces)
-        const float ta = (ez - sz) / (bz0 - sz), tb = (ez - sz) / (bz1 - sz);
-        const float xa0 = fmaf(bx0 - sx, ta, sx), xa1 = fmaf(bx1 - sx, ta, sx);
-        const float xb0 = fmaf(bx0 - sx, tb, sx), xb1 = fmaf(bx1 - sx, tb, sx);
-        const float ya0 = fmaf(by0 - sy, ta, sy), ya1 = fmaf(by1 - sy, ta, sy);
-        const float yb0 = fmaf(by0 - sy, tb, sy), yb1 = fmaf(by1 - sy, tb, sy);
-        const float xmin = fminf(fminf(xa0, xa1), fminf(xb0, xb1));
-        const float xmax = fmaxf(fmaxf(xa0, xa1), fmaxf(xb0, xb1));
-        const float ymin = fminf(fminf(ya0, ya1), fminf(yb0, yb1));
-        const float ymax = fmaxf(fmaxf(ya0, ya1), fmaxf(yb0, yb1));
-        // bins whose centres lie in [min, max]; floor/ceil widen the range by
-        // up to one bin, so rounding of the shadow cannot drop a candidate
-        // (extra candidates clip to length 0)
-        int i0 = (int)floorf((xmin - g.x_d0) / g.dbx - 0.5f);
-        int i1 = (int)ceilf((xmax - g.x_d0) / g.dbx - 0.5f);
-        int j0 = (int)floorf((ymin - g.y_d0) / g.dby - 0.5f);
-        int j1 = (int)ceilf((ymax - g.y_d0) / g.dby - 0.5f);
-        i0 = max(i0, 0);
-        j0 = max(j0, 0);
-        i1 = min(i1, g.nbx - 1);
-        j1 = min(j1, g.nby - 1);
-        float acc = 0.0f;
-        for (int j = j0; j <= j1; ++j) {
-            const float ey = bin_centre(g.y_d0, j, g.dby);
-            float ty0 = -3.0e38f, ty1 = 3.0e38f;
-            slab(ey, sy, by0, by1, ty0, ty1);
-            slab(ez, sz, bz0, bz1, ty0, ty1);
-            if (!(ty0 < ty1)) continue;
-            const float ddy = ey - sy, ddz = ez - sz;
-            const float dyz = fmaf(ddz, ddz, ddy * ddy);
-            for (int i = i0; i <= i1; ++i) {
-                const float ex = bin_centre(g.x_d0, i, g.dbx);
-                float t0 = ty0, t1 = ty1;
-                slab(ex, sx, bx0, bx1, t0, t1);
-                if (t0 < t1) {
-                    const float ddx = ex - sx;
-                    const float len = sqrtf(fmaf(ddx, ddx, dyz));
-                    acc = fmaf((t1 - t0) * len, P((uint32_t)a * nbin + (uint32_t)j * g.nbx + i),
-                               acc);
-                }
-            }
-        }
-        total = fmaf(cos_weight ? S.c[a] : 1.0f, acc, total);
-    }
+    const VolReader<TP> P(prj, (uint32_t)n_slots * g.nbx * g.nby);
+    const float total = matched_voxel<TP>(g, S, n_slots, cos_weight, P, ix, iy, iz);
     const int64_t vi = ((int64_t)iz * g.ny + iy) * g.nx + ix;
     img[vi] = img[vi] + total;
 }

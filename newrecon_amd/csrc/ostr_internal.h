@@ -310,6 +310,13 @@ hipError_t launch_momentum_step(float* x_next, const float* z, const float* z_pr
                                 hipStream_t st);
 hipError_t launch_line_integrals(float* p, const float* y, const float* b, int64_t n,
                                  hipStream_t st);
+// Fused OSTR subset update with the exact transpose (hsub detector-row-major,
+// FP32 image and h).
+hipError_t launch_update_fused_matched(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                                       const float* x_in, float* x_out, const float* d,
+                                       const float* hsub, float beta, float capl_scale,
+                                       int potential, float delta, float rho, int guard,
+                                       hipStream_t st);
 // ReconDBT SART / ML (SART_dbt.m, ML_dbt.m) elementwise steps.
 hipError_t launch_rdbt_sart_residual(float* y, const float* p, const float* est,
                                      const float* l, float thresh, int64_t n, hipStream_t st);

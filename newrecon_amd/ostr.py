@@ -96,6 +96,8 @@ def lib():
         "ostr_fproj_4r": [vp, vp, vp, vp, i32, i32, vp],
         "ostr_line_integrals": [vp, vp, vp, vp, i64, vp],
         "ostr_momentum_init": [vp, vp, vp],
+        "ostr_update_fused_matched": [vp, vp, vp, vp, vp, i32, i32, f32, f32, i32, f32, f32, i32,
+                                      vp],
         "ostr_momentum_step": [vp, vp, vp, vp, vp, vp, vp],
         "ostr_rdbt_sart_residual": [vp, vp, vp, vp, vp, f32, i64, vp],
         "ostr_rdbt_sart_update": [vp, vp, vp, vp, f32, i32, i64, vp],
@@ -314,6 +316,18 @@ class DBTSystem:
         self._check(self._L.ostr_update_fused(
             self._h, _p(x_in), _p(x_out), _dt(x_in), _p(d), _p(hsub), _dt(hsub), a0, a1, beta,
             capl_scale, POTENTIALS[potential], delta, rho, int(guard), self._stream()))
+        return x_out
+
+    def update_fused_matched(self, x_in, x_out, d, hsub, a0, a1, beta, capl_scale, potential,
+                             delta, z_xy_ratio=None, guard=True):
+        """update_fused with the exact transpose: x_out = x_in - (M G^T h + beta RD) /
+        (d + 2 beta RDD); FP32, h detector-row-major."""
+        rho = self.geom.dz / self.geom.dx if z_xy_ratio is None else z_xy_ratio
+        for t, what in ((x_in, "x_in"), (x_out, "x_out"), (d, "d"), (hsub, "h")):
+            _f32(t, what)
+        self._check(self._L.ostr_update_fused_matched(
+            self._h, _p(x_in), _p(x_out), _p(d), _p(hsub), a0, a1, beta, capl_scale,
+            POTENTIALS[potential], delta, rho, int(guard), self._stream()))
         return x_out
 
     # -- reference names (Tomo_OSTR.cu launch wrappers, gold.cpp host ops) ----
@@ -578,10 +592,11 @@ class OSTR:
         if self.cfg.backprojector not in BACKPROJECTORS:
             raise ValueError(f"unknown backprojector {self.cfg.backprojector!r}")
         self.matched = self.cfg.backprojector == "matched"
-        # the fused kernels implement the SBP update; the matched update runs
-        # launch by launch (forward, h, exact transpose, prior, update)
-        self.fused = self.cfg.fused and not self.matched
-        if not self.fused and (img_dtype != torch.float32 or prj_dtype != torch.float32):
+        # fused: two launches per subset (forward + h, then backprojection +
+        # prior + update); the matched fused update is FP32
+        self.fused = self.cfg.fused
+        if (not self.fused or self.matched) and (img_dtype != torch.float32
+                                                 or prj_dtype != torch.float32):
             raise ValueError("BF16/FP16 storage is implemented by the fused SBP kernels only")
         self.sys = DBTSystem(geom, angles, device=device)
         self.dev = self.sys.device
@@ -644,7 +659,7 @@ class OSTR:
         """gamma, yry, d = SBP(gamma*yry) (natural order), then the OS regroup."""
         s, cfg = self.sys, self.cfg
         nv = self.sys.n_views
-        if self.fused:
+        if self.fused and not self.matched:
             self.d = s.precompute_d(self.y_nat, self.r_nat, cfg.gamma_mode)
         elif self.matched:  # d = G^T(yry * G 1)
             gam = s.fproj(s.volume(fill=1.0), 0, nv, cos_weight=True)
@@ -683,8 +698,14 @@ class OSTR:
         if self.fused:
             self._timed("fproj_h", s.fproj_h, self.x, self.y[sl], self.b[sl], r_sub, a0, a1,
                         hsub=self.h)
-            self._timed("update_fused", s.update_fused, self.x, self.x_next, self.d, self.h, a0,
-                        a1, cfg.beta, float(self.M), cfg.prior, cfg.delta, self.rho, cfg.guard)
+            if self.matched:
+                self._timed("update_fused_matched", s.update_fused_matched, self.x, self.x_next,
+                            self.d, self.h, a0, a1, cfg.beta, float(self.M), cfg.prior,
+                            cfg.delta if cfg.prior != "q2" else 1.0, self.rho, cfg.guard)
+            else:
+                self._timed("update_fused", s.update_fused, self.x, self.x_next, self.d, self.h,
+                            a0, a1, cfg.beta, float(self.M), cfg.prior, cfg.delta, self.rho,
+                            cfg.guard)
             self.x, self.x_next = self.x_next, self.x
         else:
             s.fprojectCB_1R_GPU_OSTR_cos(self.x, self.line, a0, a1)
@@ -730,14 +751,19 @@ class OSTR:
         self._spare, self.z_prev, self.x = self.z_prev, z, xn
 
     def _one_iteration(self):
-        if self.cfg.momentum:  # x_k, the point this pass starts from
-            if getattr(self, "_xk", None) is None:
+        # x_k, the point this pass starts from: with one fused subset it is
+        # still in the input buffer of the update after the pass
+        keep_in_place = self.fused and self.M == 1
+        if self.cfg.momentum and not keep_in_place:
+            if getattr(self, "_xk", None) is None or self._xk is getattr(self, "x_next", None):
                 self._xk = torch.empty_like(self.x)
             self._xk.copy_(self.x)
         for m in range(self.M):
             self._subset(m)
         self.iteration += 1
         if self.cfg.momentum:
+            if keep_in_place:
+                self._xk = self.x_next
             self._momentum_step()
 
     def _buffers(self):

@@ -34,7 +34,7 @@ def test_graph_replay_equals_eager(small_case, dev, backprojector, M, momentum):
         assert torch.isfinite(junk)
     if momentum:
         assert graph.restarts == eager.restarts
-    assert graph._graph is not None and graph._graph_iters in (1, 2, 3, 6)
+    assert graph._graph is not None and 1 <= graph._graph_iters <= 12
 
 
 def test_device_momentum_matches_host_formula(small_case, dev):

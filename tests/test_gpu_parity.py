@@ -215,19 +215,20 @@ def test_ostr_small_per_iteration_parity(small_case, dev, fused, oracle_lib):
     assert max(errs) <= 1e-4, errs
 
 
+@pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("M", [1, 3])
-def test_ostr_matched_per_iteration_parity(small_case, dev, M, oracle_lib):
+def test_ostr_matched_per_iteration_parity(small_case, dev, M, fused, oracle_lib):
     """OS-SQS with the exact transpose (d = G^T(yry G 1), capL = M G^T h):
     <= 1e-4 rel RMSE against the FP32 oracle (clip transpose) after every
     iteration, and the FP64 objective decreases monotonically for M = 1."""
     s, x, y, b = small_case
     n = 6
     cfg = OSTRConfig(n_subsets=M, beta=1e3, delta=5e-4, prior="hyperbola",
-                     backprojector="matched")
+                     backprojector="matched", fused=fused)
     _, snaps = oracle_lib.run(s.geom, s.angles, y, b, n_subsets=M, n_iters=n, beta=1e3,
                               delta=5e-4, potential=2, snapshots=True, backprojector=1)
     rec = OSTR(s.geom, s.angles, y, b, cfg=cfg, device=dev)
-    assert not rec.fused
+    assert rec.fused == fused
     errs, objs = [], []
 
     def cb(it, xg):
@@ -264,6 +265,20 @@ def test_fused_equals_unfused(small_case, dev):
         cfg = OSTRConfig(n_subsets=3, n_iters=3, fused=fused)
         out.append(OSTR(s.geom, s.angles, y, b, cfg=cfg, device=dev).iterate(3).cpu().numpy())
     assert rel_max(out[1], out[0]) <= 1e-6
+
+
+@pytest.mark.parametrize("prior,M,momentum", [("hyperbola", 3, False), ("lange", 1, True),
+                                              ("q2", 3, False)])
+def test_fused_matched_equals_unfused(small_case, dev, prior, M, momentum):
+    """The fused matched update (exact transpose + prior + update in one
+    launch) equals ostr_bproj_matched + ostr_prior + ostr_update_est bit for bit."""
+    s, x, y, b = small_case
+    out = []
+    for fused in (False, True):
+        cfg = OSTRConfig(n_subsets=M, prior=prior, backprojector="matched", momentum=momentum,
+                         fused=fused)
+        out.append(OSTR(s.geom, s.angles, y, b, cfg=cfg, x0="bp", device=dev).iterate(4))
+    assert torch.equal(out[0], out[1])
 
 
 @pytest.mark.parametrize("prior,gmode", [("lange", "ray_length"), ("q2", "row_sum")])
