@@ -173,3 +173,32 @@ def test_sart_errors(dev):
         sysg.sart_fproj_diff(img, rows, 0, 2, diff_rows=rows)
     with pytest.raises(OSTRError):
         sysg.prior_sart(img, 1e-3, prior=img)
+
+
+def test_sart_ragged_sizes_and_empty_ranges(dev, oracle_lib):
+    """A volume and detector that fill no tile (13x7x5 voxels, 37x9 bins, 3 views):
+    the SART forward, the update with prior and the fused loop against the
+    oracle; empty slot ranges are accepted and change nothing."""
+    from newrecon_amd.geometry import scan_tiny
+    s = scan_tiny(nx=13, ny=7, nz=5, nbx=37, nby=9, n_views=3)
+    g = s.geom
+    rng = np.random.default_rng(17)
+    x = (rng.random(g.vol_shape) * 0.02 + 0.005).astype(np.float32)
+    p = oracle_lib.fproj(g, s.angles, x, 0, 3, cos_weight=False)
+    S = DBTSystem(g, s.angles, device=dev)
+    prj, norm = S.fproj_sart(T(x, dev), 0, 3)
+    assert rel_max(prj.cpu().numpy(), p) <= 1e-6
+    ref, snaps = oracle_lib.run_sart(g, s.angles, p, n_subsets=3, n_iters=3, lam=0.9, beta=-1e-3,
+                                     delta=1e-3, snapshots=True)
+    for fused in (True, False):
+        rec = OSSART(g, s.angles, p, n_subsets=3, lam=0.9, beta=-1e-3, delta=1e-3, fused=fused,
+                     device=dev)
+        errs = []
+        rec.iterate(3, callback=lambda it, xi: errs.append(rel_rmse(xi.cpu().numpy(),
+                                                                       snaps[it - 1])))
+        assert max(errs) <= 1e-4, (fused, errs)
+    img = T(x, dev)
+    before = img.clone()
+    S.bproj_sart(S.sinogram(1), 2, 2, img, 1.0)             # empty range
+    S.fproj_sart(img, 1, 1, S.sinogram(1), S.sinogram(1))   # empty range
+    assert torch.equal(img, before)
