@@ -271,6 +271,16 @@ int ostr_bproj_sart_r(ostr_handle* h, float* img, const float* diff, const float
  * errors. */
 int ostr_prior_sart(ostr_handle* h, float* prior, const float* img, float delta, void* stream);
 
+/* FDK-style filtering for a DBT FBP (ReconDBT/fbp_dbt.m's role; its IRT
+ * feldkamp is not available, so this is the textbook form on the flat
+ * detector): over slots [a0,a1) in the reference layout [slot][nbx][nby],
+ * q[a][i][j] = sum_k taps[k] cw(i', j) p[a][i'][j], i' = i + k - (ntaps-1)/2,
+ * cw = D / |S - E| the cosine of the ray (S the slot's source, E the bin centre,
+ * D the source height above the detector plane), samples off the detector 0.
+ * ntaps odd; taps in device memory. */
+int ostr_fbp_filter(ostr_handle* h, float* q, const float* p, const float* taps, int32_t ntaps,
+                    int32_t a0, int32_t a1, void* stream);
+
 /* ---- ReconDBT SART and ML (ReconDBT/SART_dbt.m, ML_dbt.m) on this build's
  * projector pair: G = ostr_fproj without cos, G^T = ostr_bproj_matched without
  * cos. The MATLAB package applies IRT system matrices; these are its per-view

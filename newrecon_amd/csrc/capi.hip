@@ -395,6 +395,23 @@ OSTR_API int ostr_line_integrals(ostr_handle* h, float* p, const float* y, const
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_line_integrals");
 }
 
+OSTR_API int ostr_fbp_filter(ostr_handle* h, float* q, const float* p, const float* taps,
+                             int32_t ntaps, int32_t a0, int32_t a1, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_range(h, a0, a1, "ostr_fbp_filter"))) return rc;
+    if (!q || !p || !taps || q == p) return fail(h, OSTR_E_ARG, "ostr_fbp_filter: bad buffers");
+    if (ntaps <= 0 || ntaps % 2 == 0)
+        return fail(h, OSTR_E_ARG, "ostr_fbp_filter: ntaps must be odd and positive");
+    if ((rc = use_device(h))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n = (int64_t)(a1 - a0) * h->geom.nbx * h->geom.nby;
+    if ((rc = ensure_work(h, n * sizeof(float), st, "ostr_fbp_filter"))) return rc;
+    hipError_t e = ostr::launch_fbp_filter(h->dg, h->views_dev, a0, a1, taps, ntaps, p,
+                                           (float*)h->work, q, st);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_fbp_filter");
+}
+
 // ---- ReconDBT SART / ML steps ------------------------------------------------
 
 OSTR_API int ostr_rdbt_sart_residual(ostr_handle* h, float* y, const float* p, const float* est,

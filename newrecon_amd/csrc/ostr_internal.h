@@ -308,6 +308,10 @@ hipError_t launch_momentum_init(double* state, hipStream_t st);
 hipError_t launch_momentum_step(float* x_next, const float* z, const float* z_prev,
                                 const float* x_k, int64_t n, double* partials, double* state,
                                 hipStream_t st);
+// FDK-style cosine weighting and row filtering of views (DBT FBP).
+hipError_t launch_fbp_filter(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                             const float* taps, int ntaps, const float* p, float* pw, float* q,
+                             hipStream_t st);
 hipError_t launch_line_integrals(float* p, const float* y, const float* b, int64_t n,
                                  hipStream_t st);
 // Fused OSTR subset update with the exact transpose (hsub detector-row-major,

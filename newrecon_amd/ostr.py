@@ -95,6 +95,7 @@ def lib():
         "ostr_sart_fproj_diff": [vp, vp, vp, vp, i32, i32, i32, vp],
         "ostr_fproj_4r": [vp, vp, vp, vp, i32, i32, vp],
         "ostr_line_integrals": [vp, vp, vp, vp, i64, vp],
+        "ostr_fbp_filter": [vp, vp, vp, vp, i32, i32, i32, vp],
         "ostr_momentum_init": [vp, vp, vp],
         "ostr_update_fused_matched": [vp, vp, vp, vp, vp, i32, i32, f32, f32, i32, f32, f32, i32,
                                       vp],
@@ -486,6 +487,17 @@ class DBTSystem:
         """Tomo_OSTR.cu:2398: d_prjbuf [S][nby][nbx] = merged 2x2 sub-bin sums."""
         return self.fproj_4r(d_objbuf, angleStart, angleEnd, d_prjbuf, d_4R_prjbuf)
 
+    def fbp_filter(self, p, taps, a0, a1, q=None):
+        """Cosine-weighted views filtered along detector x with odd-length taps."""
+        q = self.sinogram(a1 - a0) if q is None else q
+        for t, what in ((p, "p"), (taps, "taps"), (q, "q")):
+            _f32(t, what)
+        if taps.numel() % 2 == 0:
+            raise ValueError("FBP filter taps must have odd length")
+        self._check(self._L.ostr_fbp_filter(self._h, _p(q), _p(p), _p(taps), taps.numel(), a0,
+                                            a1, self._stream()))
+        return q
+
     def line_integrals(self, y, b, p=None):
         """p = log(b / max(y, 1))."""
         p = torch.empty_like(y) if p is None else p
@@ -615,6 +627,12 @@ class OSTR:
             self.x = self.sys.volume(img_dtype)
         elif isinstance(x0, str) and x0 == "bp":
             self.x = self.sys.bp_init(y=self.y_nat, b=self.b_nat).to(img_dtype).contiguous()
+        elif isinstance(x0, str) and x0 == "fbp":
+            # FDK-style FBP (Hann window, the fbp_dbt.m default), negatives set to 0
+            from .fbp import fbp_dbt
+            p = self.sys.line_integrals(self.y_nat, self.b_nat)
+            img = fbp_dbt(geom, angles, p, "hann", device=self.dev)
+            self.x = self.sys.extrapolate(torch.empty_like(img), img, img, 0.0).to(img_dtype)
         elif isinstance(x0, str) and x0 in ("sart", "ml"):
             # ReconDBT's iterative initialisers from the BP image, with the
             # example's settings (dbt_recon_circular_example.m:92, :100)
