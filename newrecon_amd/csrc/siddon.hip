@@ -173,32 +173,41 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
         const float thr = stop_threshold(lmax);
         bool active = at < thr;
         while (active) {
+            // branch-free: a lane that stops inside the batch keeps its state
+            // through the remaining steps (every update is a select on act),
+            // so the per-step flags stay lane masks and no step needs a branch
             float seg[WALK_BATCH];
             uint32_t off[WALK_BATCH];
             bool ok[WALK_BATCH];
 #pragma unroll
             for (int k = 0; k < WALK_BATCH; ++k) {
-                ok[k] = false;
-                seg[k] = 0.0f;
-                off[k] = 0;
-                if (active) {
-                    seg[k] = (at - cur) * len;
-                    ok[k] = inside();
-                    off[k] = vid;
-                    advance();
-                    active = at < thr;
-                }
+                const bool act = active;
+                seg[k] = (at - cur) * len;
+                ok[k] = act && inside();
+                off[k] = ok[k] ? vid : 0u;
+                const bool sx = act && at == X.next;
+                const bool sy = act && at == Y.next;
+                const bool sz = act && at == Z.next;
+                X.p = sx ? X.p + X.step : X.p;
+                Y.p = sy ? Y.p + Y.step : Y.p;
+                Z.p = sz ? Z.p + Z.step : Z.p;
+                X.next = sx ? X.next + X.inc : X.next;
+                Y.next = sy ? Y.next + Y.inc : Y.next;
+                Z.next = sz ? Z.next + Z.inc : Z.next;
+                vid += (sx ? sxs : 0u) + (sy ? sys : 0u) + (sz ? szs : 0u);
+                const float nat = fminf(fminf(X.next, Y.next), Z.next);
+                cur = act ? at : cur;
+                at = act ? nat : at;
+                active = act && at < thr;
             }
             float v[WALK_BATCH];
 #pragma unroll
-            for (int k = 0; k < WALK_BATCH; ++k) v[k] = (!ONES && ok[k]) ? img(off[k]) : 1.0f;
+            for (int k = 0; k < WALK_BATCH; ++k) v[k] = ONES ? 1.0f : img(off[k]);
 #pragma unroll
             for (int k = 0; k < WALK_BATCH; ++k) {
-                if (ok[k]) {
-                    sum = fmaf(v[k], seg[k], sum);
-                    if (NORM) *norm = fmaf(1.0f, seg[k], *norm);
-                    if (COUNT && seg[k] > 0.0f) ++nseg;
-                }
+                sum = ok[k] ? fmaf(v[k], seg[k], sum) : sum;
+                if (NORM) *norm = ok[k] ? fmaf(1.0f, seg[k], *norm) : *norm;
+                if (COUNT && ok[k] && seg[k] > 0.0f) ++nseg;
             }
         }
     } else {
