@@ -27,6 +27,8 @@
 //   x_out = max(0, x - (capL + b RD)/(d + 2 b RDD))   (update_est :265-295)
 // reading x_in and writing x_out (double-buffered, so the prior sees the
 // pre-update image as the reference's sequential order does).
+#include <type_traits>
+
 #include "matched_device.h"
 #include "prior_device.h"
 
@@ -40,6 +42,8 @@ namespace {
 constexpr int CASE_ROWS = 8;
 constexpr int CASE_WAVES = 4;
 constexpr int TILE_X = 64;
+// prior_rows modes (by_prior_mode)
+constexpr int PRIOR_INTERIOR = 0, PRIOR_EDGE = 1, PRIOR_GENERIC = 2;
 
 __device__ __forceinline__ float cell_centre(float origin, int i, float size) {
     return (float)((double)origin + ((double)i + 0.5) * (double)size);
@@ -242,16 +246,23 @@ __global__ __launch_bounds__(256) void sart_update_kernel(DevGeom g, SlotTab S, 
 
 // Prior terms of the ROWS voxels (ix, iy0 + r, iz), r = 0..ROWS-1, from a
 // sliding 3x3x3 window: each step loads the 9 values (3 slices x 3 columns) of
-// the next row, so a thread reads 9*(ROWS+2) values instead of 26*ROWS. Terms
-// are summed per voxel in the reference's order (dz outer, dy, dx inner),
-// rdd from 0 and rd from init(r); then epi(r, vi, x_centre, rdd, rd) is called
-// for each voxel inside the volume. CHECKED: the tile touches the volume
-// boundary; a neighbour outside the volume contributes t = 0 (it is replaced
-// by the voxel itself, Tomo_OSTR.cu:238-241).
-template <int POT, bool CHECKED, typename TI, int ROWS, typename Init, typename Epi>
+// the next row, so a thread reads 9*(ROWS+2) values instead of 26*ROWS. The
+// 26 neighbours are visited in the reference's order (dz outer, dy, dx inner)
+// and summed per weight class: a[c] = sum r(t), b[c] = sum r(t) t (prior_r:
+// omega = |delta| r with |delta| in the class weights), then
+// rdd = sum_c w_c a[c], rd = init(r) + sum_c w_c b[c], the same terms
+// w_k omega(t) and w_k omega(t) t as the reference's per-neighbour sums with
+// one weight multiply per class instead of one per neighbour and no t/delta;
+// then epi(r, vi, x_centre, rdd, rd) is called for each voxel inside the
+// volume. MODE (prior_mode): PRIOR_INTERIOR, the window stays inside the
+// volume; otherwise a neighbour outside the volume contributes t = 0 (it is
+// replaced by the voxel itself, Tomo_OSTR.cu:238-241), and PRIOR_GENERIC also
+// takes k < 0.
+template <int POT, int MODE, typename TI, int ROWS, typename Init, typename Epi>
 __device__ __forceinline__ void prior_rows(const DevGeom& g, const VolReader<TI>& X,
                                            const TileCoord& c, const PriorClass& W,
                                            float inv_delta, Init init, Epi epi) {
+    constexpr bool CHECKED = MODE != PRIOR_INTERIOR;
     const int ix = c.ix, iy0 = c.iy0, iz = c.iz;
     const uint32_t nx = g.nx, plane = (uint32_t)g.nx * g.ny;
     bool cx[3], cz[3];
@@ -287,7 +298,9 @@ __device__ __forceinline__ void prior_rows(const DevGeom& g, const VolReader<TI>
         const float xc = win[1][1][1];
         const int iy = iy0 + r;
         const bool store = lane_ok && iy < g.ny;
-        float rdd = 0.0f, rd = init(r, store);
+        float a[5], b[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) a[k] = b[k] = 0.0f;
 #pragma unroll
         for (int dz = 0; dz < 3; ++dz)
 #pragma unroll
@@ -298,10 +311,17 @@ __device__ __forceinline__ void prior_rows(const DevGeom& g, const VolReader<TI>
                     if (nidx == 13) continue;
                     float t = xc - win[dy][dz][dx];
                     if (CHECKED) t = ok[dy][dz][dx] ? t : 0.0f;
-                    const float e = W.w[prior_class(nidx)] * prior_omega<POT>(t, inv_delta);
-                    rdd = rdd + e;
-                    rd = fmaf(e, t, rd);
+                    const float rt = prior_r<POT, MODE != PRIOR_GENERIC>(t, W.k, inv_delta);
+                    const int k = prior_class(nidx);
+                    a[k] = a[k] + rt;
+                    b[k] = fmaf(rt, t, b[k]);
                 }
+        float rdd = 0.0f, rd = init(r, store);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            rdd = fmaf(W.w[k], a[k], rdd);
+            rd = fmaf(W.w[k], b[k], rd);
+        }
         if (store) epi(r, (uint32_t)iz * plane + (uint32_t)iy * nx + (uint32_t)ix, xc, rdd, rd);
 #pragma unroll
         for (int dz = 0; dz < 3; ++dz)
@@ -317,17 +337,35 @@ __device__ __forceinline__ void prior_rows(const DevGeom& g, const VolReader<TI>
     }
 }
 
+// Calls f(std::integral_constant<int, MODE>) with the prior_rows mode of the
+// wave (wave-uniform): PRIOR_INTERIOR when the 3x3x3 neighbourhoods of its R
+// rows stay inside the volume and k >= 0, PRIOR_EDGE when only k >= 0,
+// PRIOR_GENERIC otherwise.
+template <int R, typename F>
+__device__ __forceinline__ void by_prior_mode(const DevGeom& g, const TileCoord& c,
+                                              const PriorClass& W, F f) {
+    const int x_first = c.ix - (threadIdx.x & 63);
+    const bool inside = x_first >= 1 && x_first + 64 < g.nx && c.iy0 >= 1 &&
+                        c.iy0 + R < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
+    if (W.k < 0.0f)
+        f(std::integral_constant<int, PRIOR_GENERIC>());
+    else if (inside)
+        f(std::integral_constant<int, PRIOR_INTERIOR>());
+    else
+        f(std::integral_constant<int, PRIOR_EDGE>());
+}
+
 // Prior terms and SQS update of the ROWS voxels (the OSTR epilogue of the
 // fused update): capL = M (0 + total), x = max(0, x - (capL + beta rd) /
 // (d + 2 beta rdd)); guard: keep x where the denominator is 0.
-template <int POT, bool CHECKED, typename TI, int ROWS>
+template <int POT, int MODE, typename TI, int ROWS>
 __device__ __forceinline__ void update_rows(const DevGeom& g, const VolReader<TI>& X,
                                             const VolWriter<TI>& XO, const VolReader<float>& D,
                                             const TileCoord& c, const float total[ROWS],
                                             const PriorClass& W, float inv_delta, float beta,
                                             float capl_scale, int guard) {
     const float two_beta = 2.0f * beta;
-    prior_rows<POT, CHECKED, TI, ROWS>(
+    prior_rows<POT, MODE, TI, ROWS>(
         g, X, c, W, inv_delta, [](int, bool) { return 0.0f; },
         [&](int r, uint32_t vi, float xc, float rdd, float rd) {
             // capL: 0 + total, then * M (float), as the reference host does
@@ -365,15 +403,10 @@ __global__ __launch_bounds__(256) void huber_prior_kernel(DevGeom g, PriorClass 
     const uint32_t row0 = ((uint32_t)c.iz * g.ny + (uint32_t)c.iy0) * g.nx + (uint32_t)c.ix;
     auto init = [&](int r, bool store) { return store ? PR(row0 + (uint32_t)r * g.nx) : 0.0f; };
     auto epi = [&](int, uint32_t vi, float, float, float rd) { PW(vi, rd); };
-    const int x_first = c.ix - (threadIdx.x & 63);
-    const bool interior = x_first >= 1 && x_first + 64 < g.nx && c.iy0 >= 1 &&
-                          c.iy0 + CASE_ROWS < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
-    if (interior)
-        prior_rows<OSTR_PRIOR_HYPERBOLA, false, float, CASE_ROWS>(g, X, c, W, inv_delta, init,
-                                                                 epi);
-    else
-        prior_rows<OSTR_PRIOR_HYPERBOLA, true, float, CASE_ROWS>(g, X, c, W, inv_delta, init,
-                                                                epi);
+    by_prior_mode<CASE_ROWS>(g, c, W, [&](auto m) {
+        prior_rows<OSTR_PRIOR_HYPERBOLA, decltype(m)::value, float, CASE_ROWS>(g, X, c, W,
+                                                                              inv_delta, init, epi);
+    });
 }
 
 // Fused OS-SART subset update (the backprj_OSSART[_R] update with the Huber
@@ -406,15 +439,10 @@ __global__ __launch_bounds__(256) void sart_update_fused_kernel(
     };
     if (PRIOR) {
         auto init = [](int, bool) { return 0.0f; };
-        const int x_first = c.ix - (threadIdx.x & 63);
-        const bool interior = x_first >= 1 && x_first + 64 < g.nx && c.iy0 >= 1 &&
-                              c.iy0 + CASE_ROWS < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
-        if (interior)
-            prior_rows<OSTR_PRIOR_HYPERBOLA, false, float, CASE_ROWS>(g, X, c, W, inv_delta,
-                                                                     init, epi);
-        else
-            prior_rows<OSTR_PRIOR_HYPERBOLA, true, float, CASE_ROWS>(g, X, c, W, inv_delta,
-                                                                    init, epi);
+        by_prior_mode<CASE_ROWS>(g, c, W, [&](auto m) {
+            prior_rows<OSTR_PRIOR_HYPERBOLA, decltype(m)::value, float, CASE_ROWS>(
+                g, X, c, W, inv_delta, init, epi);
+        });
     } else {
         if (c.ix >= g.nx) return;
         const uint32_t plane = (uint32_t)g.nx * g.ny;
@@ -445,16 +473,10 @@ __device__ __forceinline__ void update_tile(const DevGeom& g, const SlotTab& S, 
     const VolReader<float> D(d, nvox);
     float total[R];
     sbp_rows<TP, R>(g, S, n_slots, P, c.ix, c.iy0, c.iz, total);
-    // wave-uniform: does the 3x3x3 neighbourhood of the wave's rows stay inside?
-    const int x_first = c.ix - (threadIdx.x & 63);
-    const bool interior = x_first >= 1 && x_first + 64 < g.nx && c.iy0 >= 1 &&
-                          c.iy0 + R < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
-    if (interior)
-        update_rows<POT, false, TI, R>(g, X, XO, D, c, total, W, inv_delta, beta, capl_scale,
-                                       guard);
-    else
-        update_rows<POT, true, TI, R>(g, X, XO, D, c, total, W, inv_delta, beta, capl_scale,
-                                      guard);
+    by_prior_mode<R>(g, c, W, [&](auto m) {
+        update_rows<POT, decltype(m)::value, TI, R>(g, X, XO, D, c, total, W, inv_delta, beta,
+                                                    capl_scale, guard);
+    });
 }
 
 // Fused OSTR update with the exact transpose (the matched counterpart of
@@ -484,15 +506,10 @@ __global__ __launch_bounds__(256) void update_fused_matched_kernel(
         total[r] = (c.ix < g.nx && c.iy0 + r < g.ny)
                        ? matched_voxel<float>(g, S, n_slots, 1, P, c.ix, c.iy0 + r, c.iz)
                        : 0.0f;
-    const int x_first = c.ix - (threadIdx.x & 63);
-    const bool interior = x_first >= 1 && x_first + 64 < g.nx && c.iy0 >= 1 &&
-                          c.iy0 + R < g.ny && c.iz >= 1 && c.iz + 1 < g.nz;
-    if (interior)
-        update_rows<POT, false, float, R>(g, X, XO, D, c, total, W, inv_delta, beta, capl_scale,
-                                          guard);
-    else
-        update_rows<POT, true, float, R>(g, X, XO, D, c, total, W, inv_delta, beta, capl_scale,
-                                         guard);
+    by_prior_mode<R>(g, c, W, [&](auto m) {
+        update_rows<POT, decltype(m)::value, float, R>(g, X, XO, D, c, total, W, inv_delta,
+                                                       beta, capl_scale, guard);
+    });
 }
 
 template <int POT, typename TI, typename TP>
@@ -715,7 +732,8 @@ hipError_t launch_huber_prior(const DevGeom& g, float* prior, const float* img, 
     PriorW W27;
     prior_weights(W27.w, 1.0f);
     hipLaunchKernelGGL(huber_prior_kernel, dim3(volume_tiles(g)), dim3(256), 0, st, g,
-                       prior_classes(W27), 1.0f / delta, img, prior);
+                       prior_params(W27, OSTR_PRIOR_HYPERBOLA, delta), 1.0f / delta, img,
+                       prior);
     return hipGetLastError();
 }
 
@@ -726,7 +744,7 @@ hipError_t launch_sart_update_fused(const DevGeom& g, const ViewTab* views, int 
     if (!fill_slots(S, views, a0, a1, g.src_y)) return hipErrorInvalidValue;
     PriorW W27;
     prior_weights(W27.w, 1.0f);
-    const PriorClass W = prior_classes(W27);
+    const PriorClass W = prior_params(W27, OSTR_PRIOR_HYPERBOLA, delta);
     if (beta != 0.0f)
         hipLaunchKernelGGL(sart_update_fused_kernel<true>, dim3(volume_tiles(g)), dim3(256), 0,
                            st, g, S, a1 - a0, diff_rows, x_in, x_out, W, 1.0f / delta, lambda,
@@ -751,7 +769,7 @@ hipError_t launch_update_fused_matched(const DevGeom& g, const ViewTab* views, i
     }
     PriorW W27;
     prior_weights(W27.w, rho);
-    const PriorClass W = prior_classes(W27);
+    const PriorClass W = prior_params(W27, potential, delta);
     const float inv_delta = potential == OSTR_PRIOR_Q2 ? 0.0f : 1.0f / delta;
     const dim3 grid(volume_tiles(g, MATCHED_ROWS * CASE_WAVES));
     const int n = a1 - a0;
@@ -812,7 +830,7 @@ hipError_t launch_update_fused(const DevGeom& g, const ViewTab* views, int a0, i
     if (!fill_slots(S, views, a0, a1, g.src_y)) return hipErrorInvalidValue;
     PriorW W27;
     prior_weights(W27.w, rho);
-    const PriorClass W = prior_classes(W27);
+    const PriorClass W = prior_params(W27, potential, delta);
     float inv_delta = potential == OSTR_PRIOR_Q2 ? 0.0f : 1.0f / delta;
     const int n = a1 - a0;
     switch (potential) {

@@ -1069,7 +1069,7 @@ int batch_stage(ostr_batch* bt, const ostr_batch_case* cases, int n, int prj_dty
         if (kind == 1) {
             ostr::PriorW w27;
             ostr::prior_weights(w27.w, in.z_xy_ratio);
-            o.W = ostr::prior_classes(w27);
+            o.W = ostr::prior_params(w27, potential, in.delta);
         }
     }
     const int gpc = kind == 0 ? cfg->fwd_groups_per_cu : cfg->upd_groups_per_cu;

@@ -44,8 +44,13 @@ struct PriorW {
 // same operands, so the 5 class values are the 27 weights bit for bit.
 // Class of index dz*9+dy*3+dx: in-slice (dz = 1) m-1, else 2+m, where m is the
 // number of non-zero x, y offsets.
+// k: parameter of the folded form of the sliding-window kernels (prior_r):
+// delta^2 (hyperbola) or |delta| (Lange) with |delta| folded into w, 0 for Q2,
+// -1 for a delta outside [2^-60, 2^60] (then w are the plain weights and the
+// kernels evaluate omega through q = t / delta).
 struct PriorClass {
     float w[5];
+    float k;
 };
 __host__ __device__ constexpr int prior_class(int nidx) {
     return (nidx / 9 == 1) ? ((nidx % 3 != 1) + ((nidx / 3) % 3 != 1)) - 1
@@ -54,7 +59,20 @@ __host__ __device__ constexpr int prior_class(int nidx) {
 inline PriorClass prior_classes(const PriorW& W) {
     // representatives: in-slice edge 10, in-slice diagonal 9; other slice
     // centre 4, edge 1, corner 0
-    return PriorClass{{W.w[10], W.w[9], W.w[4], W.w[1], W.w[0]}};
+    return PriorClass{{W.w[10], W.w[9], W.w[4], W.w[1], W.w[0]}, 0.0f};
+}
+// Class weights and k for a potential and delta (see PriorClass).
+inline PriorClass prior_params(const PriorW& W, int potential, float delta) {
+    PriorClass P = prior_classes(W);
+    if (potential == OSTR_PRIOR_Q2) return P;
+    const double ad = delta < 0.0f ? -(double)delta : (double)delta;
+    if (!(ad >= 0x1p-60 && ad <= 0x1p60)) {
+        P.k = -1.0f;
+        return P;
+    }
+    for (int c = 0; c < 5; ++c) P.w[c] = (float)((double)P.w[c] * ad);
+    P.k = potential == OSTR_PRIOR_LANGE ? (float)ad : (float)(ad * ad);
+    return P;
 }
 
 // One case of a batched launch (device-resident table, rebuilt per call).

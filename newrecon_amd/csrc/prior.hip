@@ -23,7 +23,7 @@ void prior_weights(float w[27], float rho) {
 namespace {
 
 template <int POT, typename TI>
-__global__ __launch_bounds__(256) void prior_kernel(DevGeom g, PriorW W, float inv_delta,
+__global__ __launch_bounds__(256) void prior_kernel(DevGeom g, PriorClass W, float inv_delta,
                                                     const TI* __restrict__ x,
                                                     float* __restrict__ rdd,
                                                     float* __restrict__ rd) {
@@ -33,14 +33,17 @@ __global__ __launch_bounds__(256) void prior_kernel(DevGeom g, PriorW W, float i
     if (ix >= g.nx || iy >= g.ny) return;
     const int64_t vi = ((int64_t)iz * g.ny + iy) * g.nx + ix;
     float a = rdd[vi], b = rd[vi];
-    prior_voxel<POT, TI>(g, x, ix, iy, iz, W, inv_delta, a, b);
+    if (W.k >= 0.0f)
+        prior_voxel<POT, true, TI>(g, x, ix, iy, iz, W, inv_delta, a, b);
+    else
+        prior_voxel<POT, false, TI>(g, x, ix, iy, iz, W, inv_delta, a, b);
     rdd[vi] = a;
     rd[vi] = b;
 }
 
 template <typename TI>
 hipError_t launch_t(const DevGeom& g, float* rdd, float* rd, const void* img, int pot,
-                    const PriorW& W, float inv_delta, hipStream_t st) {
+                    const PriorClass& W, float inv_delta, hipStream_t st) {
     dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
     switch (pot) {
         case OSTR_PRIOR_Q2:
@@ -65,8 +68,9 @@ hipError_t launch_t(const DevGeom& g, float* rdd, float* rd, const void* img, in
 
 hipError_t launch_prior(const DevGeom& g, float* rdd, float* rd, const void* img,
                         int img_dtype, int potential, float delta, float rho, hipStream_t st) {
-    PriorW W;
-    prior_weights(W.w, rho);
+    PriorW W27;
+    prior_weights(W27.w, rho);
+    const PriorClass W = prior_params(W27, potential, delta);
     float inv_delta = potential == OSTR_PRIOR_Q2 ? 0.0f : 1.0f / delta;
     switch (img_dtype) {
         case OSTR_F32:

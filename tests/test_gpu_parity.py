@@ -151,12 +151,14 @@ def test_row_major_layout_entry_points(small_case, sysm, dev, oracle_lib):
     assert np.array_equal(reg, oracle_lib.regroup(y, 3).transpose(0, 2, 1))
 
 
-@pytest.mark.parametrize("potential,code", [("q2", 0), ("lange", 1), ("hyperbola", 2)])
-def test_prior_matches_oracle(small_case, sysm, dev, potential, code, oracle_lib):
+@pytest.mark.parametrize("potential,code,delta", [
+    ("q2", 0, 5e-4), ("lange", 1, 5e-4), ("hyperbola", 2, 5e-4),
+    # outside [2^-60, 2^60]: the kernels take omega through t/delta
+    ("lange", 1, 2.0 ** -62), ("hyperbola", 2, 2.0 ** -62), ("hyperbola", 2, 2.0 ** 62)])
+def test_prior_matches_oracle(small_case, sysm, dev, potential, code, delta, oracle_lib):
     s, x, _, _ = small_case
     rng = np.random.default_rng(6)
     xn = (x + 1e-3 * rng.standard_normal(x.shape)).astype(np.float32)
-    delta = 5e-4
     rdd0 = rng.random(x.shape).astype(np.float32)
     rd0 = rng.standard_normal(x.shape).astype(np.float32)
     r_dd, r_d = oracle_lib.prior(s.geom, xn, code, delta, rdd=rdd0, rd=rd0)
@@ -267,16 +269,18 @@ def test_fused_equals_unfused(small_case, dev):
     assert rel_max(out[1], out[0]) <= 1e-6
 
 
-@pytest.mark.parametrize("prior,M,momentum", [("hyperbola", 3, False), ("lange", 1, True),
-                                              ("q2", 3, False)])
-def test_fused_matched_equals_unfused(small_case, dev, prior, M, momentum):
+@pytest.mark.parametrize("prior,M,momentum,delta", [
+    ("hyperbola", 3, False, 5e-4), ("lange", 1, True, 5e-4), ("q2", 3, False, 5e-4),
+    ("hyperbola", 3, False, 2.0 ** -62), ("lange", 3, False, 2.0 ** 62)])
+def test_fused_matched_equals_unfused(small_case, dev, prior, M, momentum, delta):
     """The fused matched update (exact transpose + prior + update in one
-    launch) equals ostr_bproj_matched + ostr_prior + ostr_update_est bit for bit."""
+    launch) equals ostr_bproj_matched + ostr_prior + ostr_update_est bit for bit,
+    also for a delta whose square leaves the normal range."""
     s, x, y, b = small_case
     out = []
     for fused in (False, True):
         cfg = OSTRConfig(n_subsets=M, prior=prior, backprojector="matched", momentum=momentum,
-                         fused=fused)
+                         fused=fused, delta=delta)
         out.append(OSTR(s.geom, s.angles, y, b, cfg=cfg, x0="bp", device=dev).iterate(4))
     assert torch.equal(out[0], out[1])
 
