@@ -8,7 +8,10 @@
 // (Tomo_OSTR.cu:612-890) in exact arithmetic. The candidate rays are the bins
 // whose centres fall in the bounding box of the voxel's 8 corners projected
 // from the source onto the detector plane; each candidate is clipped against
-// the voxel box (slab method). The sinogram is detector-row-major
+// the voxel box (slab method: the z-slab clip once per slot, the y slab once
+// per detector row, the x slab per bin; min / max are order-free, so the
+// clip is that of the three slabs taken in any order; the ray length uses the
+// hardware square root, <= 1 ulp). The sinogram is detector-row-major
 // [slot][nby][nbx], so neighbouring lanes (voxels along x) read neighbouring
 // bins.
 //
@@ -80,14 +83,17 @@ __device__ __forceinline__ float matched_voxel(const DevGeom& g, const MatchedSl
         j0 = max(j0, 0);
         i1 = min(i1, g.nbx - 1);
         j1 = min(j1, g.nby - 1);
+        // the z slab is the same for every candidate ray of the slot
+        float tz0 = -3.0e38f, tz1 = 3.0e38f;
+        matched_slab(ez, sz, bz0, bz1, tz0, tz1);
+        const float ddz = ez - sz;
         float acc = 0.0f;
         for (int j = j0; j <= j1; ++j) {
             const float ey = matched_bin_centre(g.y_d0, j, g.dby);
-            float ty0 = -3.0e38f, ty1 = 3.0e38f;
+            float ty0 = tz0, ty1 = tz1;
             matched_slab(ey, sy, by0, by1, ty0, ty1);
-            matched_slab(ez, sz, bz0, bz1, ty0, ty1);
             if (!(ty0 < ty1)) continue;
-            const float ddy = ey - sy, ddz = ez - sz;
+            const float ddy = ey - sy;
             const float dyz = fmaf(ddz, ddz, ddy * ddy);
             for (int i = i0; i <= i1; ++i) {
                 const float ex = matched_bin_centre(g.x_d0, i, g.dbx);
@@ -95,7 +101,7 @@ __device__ __forceinline__ float matched_voxel(const DevGeom& g, const MatchedSl
                 matched_slab(ex, sx, bx0, bx1, t0, t1);
                 if (t0 < t1) {
                     const float ddx = ex - sx;
-                    const float len = sqrtf(fmaf(ddx, ddx, dyz));
+                    const float len = __builtin_amdgcn_sqrtf(fmaf(ddx, ddx, dyz));
                     acc = fmaf((t1 - t0) * len, P((uint32_t)a * nbin + (uint32_t)j * g.nbx + i),
                                acc);
                 }
