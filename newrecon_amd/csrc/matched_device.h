@@ -59,11 +59,14 @@ __device__ __forceinline__ float matched_voxel(const DevGeom& g, const MatchedSl
     const float bz0 = fmaf((float)iz, g.dz, g.z_p0), bz1 = fmaf((float)(iz + 1), g.dz, g.z_p0);
     const float sy = g.src_y, ez = -g.det_r;
     const uint32_t nbin = (uint32_t)g.nbx * g.nby;
+    const float inv_dbx = 1.0f / g.dbx, inv_dby = 1.0f / g.dby;
     float total = 0.0f;
     for (int a = 0; a < n_slots; ++a) {
         const float sx = S.x0[a], sz = S.z0[a];
-        // shadow of the voxel on the detector plane (corners at the two z faces)
-        const float ta = (ez - sz) / (bz0 - sz), tb = (ez - sz) / (bz1 - sz);
+        // shadow of the voxel on the detector plane (corners at the two z faces);
+        // it only bounds the candidate range, so hardware reciprocals suffice
+        const float ta = (ez - sz) * __builtin_amdgcn_rcpf(bz0 - sz);
+        const float tb = (ez - sz) * __builtin_amdgcn_rcpf(bz1 - sz);
         const float xa0 = fmaf(bx0 - sx, ta, sx), xa1 = fmaf(bx1 - sx, ta, sx);
         const float xb0 = fmaf(bx0 - sx, tb, sx), xb1 = fmaf(bx1 - sx, tb, sx);
         const float ya0 = fmaf(by0 - sy, ta, sy), ya1 = fmaf(by1 - sy, ta, sy);
@@ -73,12 +76,15 @@ __device__ __forceinline__ float matched_voxel(const DevGeom& g, const MatchedSl
         const float ymin = fminf(fminf(ya0, ya1), fminf(yb0, yb1));
         const float ymax = fmaxf(fmaxf(ya0, ya1), fmaxf(yb0, yb1));
         // bins whose centres lie in [min, max]; floor/ceil widen the range by
-        // up to one bin, so rounding of the shadow cannot drop a candidate
+        // up to one bin, so rounding of the shadow, or of the bin coordinate
+        // q = (min - d0) / size (taken here as a product with 1 / size), cannot
+        // drop a candidate: an error below one bin moves floor(q - 0.5) to at
+        // most the first bin whose centre is >= min, and likewise for ceil
         // (extra candidates clip to length 0)
-        int i0 = (int)floorf((xmin - g.x_d0) / g.dbx - 0.5f);
-        int i1 = (int)ceilf((xmax - g.x_d0) / g.dbx - 0.5f);
-        int j0 = (int)floorf((ymin - g.y_d0) / g.dby - 0.5f);
-        int j1 = (int)ceilf((ymax - g.y_d0) / g.dby - 0.5f);
+        int i0 = (int)floorf((xmin - g.x_d0) * inv_dbx - 0.5f);
+        int i1 = (int)ceilf((xmax - g.x_d0) * inv_dbx - 0.5f);
+        int j0 = (int)floorf((ymin - g.y_d0) * inv_dby - 0.5f);
+        int j1 = (int)ceilf((ymax - g.y_d0) * inv_dby - 0.5f);
         i0 = max(i0, 0);
         j0 = max(j0, 0);
         i1 = min(i1, g.nbx - 1);
