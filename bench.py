@@ -387,6 +387,17 @@ def bench_recondbt(args, dev):
     k_bt = sum(a.elapsed_time(z) for a, z in ev) / len(ev)
     b_bt = g.n_bin * 4 + 2 * g.n_vox * 4       # one view read; volume read and written
     achieved = b_bt / (k_bt * 1e-3) / 1e9
+    # the FBP initialiser (cosine weights, Toeplitz-GEMM row filter, SBP),
+    # timed with events over the whole call after one warm-up call
+    from newrecon_amd.fbp import fbp_dbt
+    fbp_dbt(g, scan.angles, p, "hann", device=dev)
+    fa, fz = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fa.record()
+    fbp_dbt(g, scan.angles, p, "hann", device=dev)
+    fz.record()
+    torch.cuda.synchronize()
+    fbp_ms = fa.elapsed_time(fz)
+    print(f"[bench] FBP (hann) {fbp_ms:.2f} ms", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -423,6 +434,7 @@ def bench_recondbt(args, dev):
         "config": {"workload": scan.name + "-recondbt", "views": scan.n_views,
                    "sart_stepsize": 0.15, "ml_stepsize": 2.0, "x0": "BP image"},
         "ml_dbt_s_per_iter": round(s_ml, 6),
+        "fbp_ms": round(fbp_ms, 3),
         "kernel_ms": {"matched_transpose_one_view": round(k_bt, 4)},
         "roofline": {"kernel": "matched_transpose_one_view", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

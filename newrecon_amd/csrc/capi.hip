@@ -26,6 +26,7 @@ struct ostr_handle {
     ostr::ViewTab* views_dev = nullptr;
     ostr::ViewTab* views_nat_dev = nullptr;
     unsigned long long* counter = nullptr;
+    void* blas = nullptr;  // rocBLAS handle of the FBP filter, created on first use
     void* work = nullptr;     // grow-on-demand device work buffer
     size_t work_bytes = 0;
     double* dwork = nullptr;  // FP64 partial sums of the reductions (+2 results)
@@ -226,6 +227,7 @@ OSTR_API void ostr_destroy(ostr_handle* h) {
     if (h->views_dev) (void)hipFree(h->views_dev);
     if (h->views_nat_dev) (void)hipFree(h->views_nat_dev);
     if (h->counter) (void)hipFree(h->counter);
+    ostr::fbp_blas_destroy(h->blas);
     if (h->work) (void)hipFree(h->work);
     if (h->dwork) (void)hipFree(h->dwork);
     delete h;
@@ -406,10 +408,12 @@ OSTR_API int ostr_fbp_filter(ostr_handle* h, float* q, const float* p, const flo
     if ((rc = use_device(h))) return rc;
     hipStream_t st = (hipStream_t)stream;
     const int64_t n = (int64_t)(a1 - a0) * h->geom.nbx * h->geom.nby;
-    if ((rc = ensure_work(h, n * sizeof(float), st, "ostr_fbp_filter"))) return rc;
-    hipError_t e = ostr::launch_fbp_filter(h->dg, h->views_dev, a0, a1, taps, ntaps, p,
-                                           (float*)h->work, q, st);
-    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_fbp_filter");
+    const int64_t nt = (int64_t)h->geom.nbx * h->geom.nbx;
+    if ((rc = ensure_work(h, (n + nt) * sizeof(float), st, "ostr_fbp_filter"))) return rc;
+    float* pw = (float*)h->work;
+    hipError_t e = ostr::launch_fbp_filter(h->dg, h->views_dev, a0, a1, taps, ntaps, p, pw,
+                                           pw + n, q, &h->blas, st);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_fbp_filter (weights, GEMM)");
 }
 
 // ---- ReconDBT SART / ML steps ------------------------------------------------

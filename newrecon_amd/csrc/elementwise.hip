@@ -260,14 +260,10 @@ hipError_t launch_line_integrals(float* p, const float* y, const float* b, int64
     return hipGetLastError();
 }
 
-// FDK-style filtering of the views of a DBT FBP, reference layout
-// [slot][nbx][nby]. Pass 1 (fbp_weight_kernel): each sample times the cosine of
-// its ray to the detector normal, D / |S - E| (D = z0 + detector_r, the source
-// height above the detector plane). Pass 2 (fbp_conv_kernel): convolution along
-// detector x (the tube-motion direction) with the odd-length centred taps,
-//   q[a][i][j] = sum_k taps[k] pw[a][i'][j],  i' = i + k - (ntaps - 1) / 2,
-// samples off the detector 0; one thread per output bin, j fastest (coalesced
-// reads along j, the taps uniform across the wave).
+// FDK-style weighting of the views of a DBT FBP, reference layout
+// [slot][nbx][nby]: each sample times the cosine of its ray to the detector
+// normal, D / |S - E| (D = z0 + detector_r, the source height above the
+// detector plane). The row filter that follows is in fbp.hip.
 __global__ __launch_bounds__(256) void fbp_weight_kernel(DevGeom g,
                                                          const ViewTab* __restrict__ views,
                                                          int a0, int n_slots,
@@ -289,39 +285,12 @@ __global__ __launch_bounds__(256) void fbp_weight_kernel(DevGeom g,
     }
 }
 
-__global__ __launch_bounds__(256) void fbp_conv_kernel(int nbx, int nby, int n_slots,
-                                                       const float* __restrict__ taps, int ntaps,
-                                                       const float* __restrict__ pw,
-                                                       float* __restrict__ q) {
-    const int64_t nbin = (int64_t)nbx * nby;
-    const int64_t n = nbin * n_slots;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < n;
-         o += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t slot = o / nbin;
-        const int rem = (int)(o % nbin);
-        const int i = rem / nby, j = rem % nby;
-        const float* col = pw + slot * nbin + j;
-        const int c = (ntaps - 1) / 2;
-        int k0 = c - i, k1 = c + (nbx - 1 - i);
-        k0 = k0 < 0 ? 0 : k0;
-        k1 = k1 > ntaps - 1 ? ntaps - 1 : k1;
-        float acc = 0.0f;
-        for (int k = k0; k <= k1; ++k) acc = fmaf(taps[k], col[(int64_t)(i + k - c) * nby], acc);
-        q[o] = acc;
-    }
-}
-
-hipError_t launch_fbp_filter(const DevGeom& g, const ViewTab* views, int a0, int a1,
-                             const float* taps, int ntaps, const float* p, float* pw, float* q,
-                             hipStream_t st) {
+hipError_t launch_fbp_weight(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                             const float* p, float* pw, hipStream_t st) {
     const int64_t n = (int64_t)g.nbx * g.nby * (a1 - a0);
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(fbp_weight_kernel, dim3(grid_for(n)), dim3(256), 0, st, g, views, a0,
                        a1 - a0, p, pw);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fbp_conv_kernel, dim3(grid_for(n)), dim3(256), 0, st, g.nbx, g.nby,
-                       a1 - a0, taps, ntaps, pw, q);
     return hipGetLastError();
 }
 

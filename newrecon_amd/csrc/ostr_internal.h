@@ -326,10 +326,17 @@ hipError_t launch_momentum_init(double* state, hipStream_t st);
 hipError_t launch_momentum_step(float* x_next, const float* z, const float* z_prev,
                                 const float* x_k, int64_t n, double* partials, double* state,
                                 hipStream_t st);
-// FDK-style cosine weighting and row filtering of views (DBT FBP).
+// FDK-style cosine weighting of views (DBT FBP): pw = D / |S - E| . p.
+hipError_t launch_fbp_weight(const DevGeom& g, const ViewTab* views, int a0, int a1,
+                             const float* p, float* pw, hipStream_t st);
+// Cosine weighting, then the row filter along detector x as one GEMM per view
+// on the matrix cores (fbp.hip): q[a] = T pw[a], T the nbx x nbx Toeplitz
+// matrix of the centred taps (written to T_buf, nbx^2 floats). *blas holds
+// the library handle, created on first use (fbp_blas_destroy frees it).
 hipError_t launch_fbp_filter(const DevGeom& g, const ViewTab* views, int a0, int a1,
-                             const float* taps, int ntaps, const float* p, float* pw, float* q,
-                             hipStream_t st);
+                             const float* taps, int ntaps, const float* p, float* pw,
+                             float* T_buf, float* q, void** blas, hipStream_t st);
+void fbp_blas_destroy(void* blas);
 hipError_t launch_line_integrals(float* p, const float* y, const float* b, int64_t n,
                                  hipStream_t st);
 // Fused OSTR subset update with the exact transpose (hsub detector-row-major,
