@@ -522,31 +522,24 @@ __global__ __launch_bounds__(256) void update_fused_kernel(
                                                     xcd_tile(blockIdx.x, gridDim.x));
 }
 
-// Batched fused update: every case of the table in one launch. The grid is
-// either one work-group per tile or persistent (a fixed number of groups per
-// CU, each looping over its share of the tiles, see BatchWalk); tiles of case
-// c are [cases[c].upd_begin, cases[c + 1].upd_begin). Scalars of the case are
-// taken into scalar registers once per tile.
-template <int POT, typename TI, typename TP, int R, int NW>
+// Batched fused update: every case of the table in one launch. LOOP: a
+// persistent grid (a fixed number of groups per CU, each looping over its
+// share of the tiles, see BatchWalk); otherwise the grid gives every XCD at
+// least as many groups as it has tiles and each group takes at most one, so
+// the walk's state is not live across the tile (scalar registers the tile
+// would otherwise spill). The case's scalars are read from the read-only case
+// table with uniform addresses (scalar loads).
+template <int POT, typename TI, typename TP, int R, int NW, bool LOOP>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(5))) void batch_update_kernel(
     const BatchCase* __restrict__ cases, const int* __restrict__ sched, int n_cases) {
     BatchWalk walk(sched, n_cases);
     int t, c;
     while (walk.next(t, c)) {
         const BatchCase& bc = cases[c];
-        const DevGeom g = uniform_copy(bc.g);
-        const PriorClass W = uniform_copy(bc.W);
-        const int n_slots = __builtin_amdgcn_readfirstlane(bc.n_slots);
-        const int guard = __builtin_amdgcn_readfirstlane(bc.guard);
-        const float inv_delta = __int_as_float(
-            __builtin_amdgcn_readfirstlane(__float_as_int(bc.inv_delta)));
-        const float beta = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(bc.beta)));
-        const float capl_scale = __int_as_float(
-            __builtin_amdgcn_readfirstlane(__float_as_int(bc.capl_scale)));
         update_tile<POT, TI, TP, R, NW>(
-            g, bc.S, n_slots, (const TP*)uniform_ptr(bc.h), (const TI*)uniform_ptr(bc.x_in),
-            (TI*)uniform_ptr(bc.x_out), uniform_ptr(bc.d), W, inv_delta, beta, capl_scale,
-            guard, t);
+            bc.g, bc.S, bc.n_slots, (const TP*)bc.h, (const TI*)bc.x_in, (TI*)bc.x_out, bc.d,
+            bc.W, bc.inv_delta, bc.beta, bc.capl_scale, bc.guard, t);
+        if (!LOOP) break;
     }
 }
 
@@ -638,28 +631,33 @@ hipError_t fused_i(int img_dtype, int prj_dtype, const DevGeom& g, const SlotTab
 
 template <int POT, typename TP, int R, int NW>
 hipError_t batch_upd_t(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
-                       hipStream_t st) {
-    hipLaunchKernelGGL((batch_update_kernel<POT, float, TP, R, NW>), dim3(grid), dim3(64 * NW),
-                       0, st, cases, sched, n_cases);
+                       int single_pass, hipStream_t st) {
+    if (single_pass)
+        hipLaunchKernelGGL((batch_update_kernel<POT, float, TP, R, NW, false>), dim3(grid),
+                           dim3(64 * NW), 0, st, cases, sched, n_cases);
+    else
+        hipLaunchKernelGGL((batch_update_kernel<POT, float, TP, R, NW, true>), dim3(grid),
+                           dim3(64 * NW), 0, st, cases, sched, n_cases);
     return hipGetLastError();
 }
 
 template <int POT, typename TP>
 hipError_t batch_upd_s(int rows, int waves, const BatchCase* cases, const int* sched, int n_cases, int total,
-                       int grid, hipStream_t st) {
-    if (rows == 8 && waves == 4) return batch_upd_t<POT, TP, 8, 4>(cases, sched, n_cases, total, grid, st);
-    if (rows == 8 && waves == 2) return batch_upd_t<POT, TP, 8, 2>(cases, sched, n_cases, total, grid, st);
-    if (rows == 4 && waves == 4) return batch_upd_t<POT, TP, 4, 4>(cases, sched, n_cases, total, grid, st);
+                       int grid, int single, hipStream_t st) {
+    if (rows == 8 && waves == 4) return batch_upd_t<POT, TP, 8, 4>(cases, sched, n_cases, total, grid, single, st);
+    if (rows == 8 && waves == 2) return batch_upd_t<POT, TP, 8, 2>(cases, sched, n_cases, total, grid, single, st);
+    if (rows == 4 && waves == 4) return batch_upd_t<POT, TP, 4, 4>(cases, sched, n_cases, total, grid, single, st);
     return hipErrorInvalidValue;
 }
 
 template <int POT>
 hipError_t batch_upd_p(int prj_dtype, int rows, int waves, const BatchCase* cases,
-                       const int* sched, int n_cases, int total, int grid, hipStream_t st) {
+                       const int* sched, int n_cases, int total, int grid, int single,
+                       hipStream_t st) {
     switch (prj_dtype) {
-        case OSTR_F32: return batch_upd_s<POT, float>(rows, waves, cases, sched, n_cases, total, grid, st);
-        case OSTR_BF16: return batch_upd_s<POT, bf16_t>(rows, waves, cases, sched, n_cases, total, grid, st);
-        case OSTR_F16: return batch_upd_s<POT, _Float16>(rows, waves, cases, sched, n_cases, total, grid, st);
+        case OSTR_F32: return batch_upd_s<POT, float>(rows, waves, cases, sched, n_cases, total, grid, single, st);
+        case OSTR_BF16: return batch_upd_s<POT, bf16_t>(rows, waves, cases, sched, n_cases, total, grid, single, st);
+        case OSTR_F16: return batch_upd_s<POT, _Float16>(rows, waves, cases, sched, n_cases, total, grid, single, st);
     }
     return hipErrorInvalidValue;
 }
@@ -682,15 +680,15 @@ void batch_slots(SlotTab& S, ViewTab* v, const ViewTab* host_views, int a0, int 
 
 hipError_t launch_batch_update(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
                                int rows, int waves, int potential, int prj_dtype,
-                               hipStream_t st) {
+                               int single_pass, hipStream_t st) {
     if (total == 0) return hipSuccess;
     switch (potential) {
         case OSTR_PRIOR_Q2:
-            return batch_upd_p<OSTR_PRIOR_Q2>(prj_dtype, rows, waves, cases, sched, n_cases, total, grid, st);
+            return batch_upd_p<OSTR_PRIOR_Q2>(prj_dtype, rows, waves, cases, sched, n_cases, total, grid, single_pass, st);
         case OSTR_PRIOR_LANGE:
-            return batch_upd_p<OSTR_PRIOR_LANGE>(prj_dtype, rows, waves, cases, sched, n_cases, total, grid, st);
+            return batch_upd_p<OSTR_PRIOR_LANGE>(prj_dtype, rows, waves, cases, sched, n_cases, total, grid, single_pass, st);
         case OSTR_PRIOR_HYPERBOLA:
-            return batch_upd_p<OSTR_PRIOR_HYPERBOLA>(prj_dtype, rows, waves, cases, sched, n_cases, total, grid, st);
+            return batch_upd_p<OSTR_PRIOR_HYPERBOLA>(prj_dtype, rows, waves, cases, sched, n_cases, total, grid, single_pass, st);
     }
     return hipErrorInvalidValue;
 }

@@ -994,7 +994,7 @@ int batch_grid(const ostr_batch* bt, int total, int groups_per_cu) {
 int batch_stage(ostr_batch* bt, const ostr_batch_case* cases, int n, int prj_dtype,
                 int potential, const ostr_tile_cfg* cfg, int kind, hipStream_t st,
                 const char* fn, ostr::BatchCase** table, const int** sched, int* total,
-                int* grid_out, int* slot) {
+                int* grid_out, int* slot, int* single_pass) {
     if (!bt) return bfail(nullptr, OSTR_E_ARG, "%s: NULL batch", fn);
     if (!cases || !cfg) return bfail(bt, OSTR_E_ARG, "%s: NULL argument", fn);
     if (n < 1 || n > bt->max_cases)
@@ -1077,8 +1077,21 @@ int batch_stage(ostr_batch* bt, const ostr_batch_case* cases, int n, int prj_dty
         }
     }
     const int gpc = kind == 0 ? cfg->fwd_groups_per_cu : cfg->upd_groups_per_cu;
-    const int grid = batch_grid(bt, (int)tiles, gpc);
+    int grid = batch_grid(bt, (int)tiles, gpc);
     ostr::batch_schedule((int*)bt->host[k], cnt.data(), n, grid);
+    // one group per tile: widen the grid to 8 x the largest per-XCD tile count
+    // (the schedule is the same for any grid >= 8), so no group needs a second
+    // tile and the kernels run their single-pass form
+    *single_pass = 0;
+    if (gpc == 0 && grid >= 8) {
+        int mx = 0;
+        for (int x = 0; x < 8; ++x) {
+            const int len = ((const int*)bt->host[k])[x * (n + 1) + n];
+            mx = len > mx ? len : mx;
+        }
+        grid = 8 * mx;
+        *single_pass = 1;
+    }
     unsigned char* D = bt->dev + (size_t)k * bt->slot_bytes;
     e = hipMemcpyAsync(D, bt->host[k], bt->case_off + sizeof(ostr::BatchCase) * n,
                        hipMemcpyHostToDevice, st);
@@ -1153,12 +1166,12 @@ OSTR_API int ostr_batch_fproj_h(ostr_batch* bt, const ostr_batch_case* cases, in
     hipStream_t st = (hipStream_t)stream;
     ostr::BatchCase* table;
     const int* sched;
-    int total, grid, slot, rc;
+    int total, grid, slot, single, rc;
     if ((rc = batch_stage(bt, cases, n_cases, prj_dtype, 0, cfg, 0, st, "ostr_batch_fproj_h",
-                          &table, &sched, &total, &grid, &slot)))
+                          &table, &sched, &total, &grid, &slot, &single)))
         return rc;
     hipError_t e = ostr::launch_batch_project(table, sched, n_cases, total, grid,
-                                              cfg->fwd_rows, prj_dtype, st);
+                                              cfg->fwd_rows, prj_dtype, single, st);
     return batch_finish(bt, slot, e, st, "ostr_batch_fproj_h");
 }
 
@@ -1168,12 +1181,13 @@ OSTR_API int ostr_batch_update_fused(ostr_batch* bt, const ostr_batch_case* case
     hipStream_t st = (hipStream_t)stream;
     ostr::BatchCase* table;
     const int* sched;
-    int total, grid, slot, rc;
+    int total, grid, slot, single, rc;
     if ((rc = batch_stage(bt, cases, n_cases, prj_dtype, potential, cfg, 1, st,
-                          "ostr_batch_update_fused", &table, &sched, &total, &grid, &slot)))
+                          "ostr_batch_update_fused", &table, &sched, &total, &grid, &slot,
+                          &single)))
         return rc;
     hipError_t e = ostr::launch_batch_update(table, sched, n_cases, total, grid, cfg->upd_rows,
-                                             cfg->upd_waves, potential, prj_dtype, st);
+                                             cfg->upd_waves, potential, prj_dtype, single, st);
     return batch_finish(bt, slot, e, st, "ostr_batch_update_fused");
 }
 

@@ -267,11 +267,13 @@ bool batch_update_shape_ok(int rows, int waves);
 // handle's host view table (a1 - a0 <= MAX_SLOTS).
 void batch_slots(SlotTab& S, ViewTab* v, const ViewTab* host_views, int a0, int a1,
                  float src_y);
+// single_pass: the grid gives every XCD at least as many groups as it has
+// tiles (each group takes at most one); otherwise the groups loop.
 hipError_t launch_batch_project(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
-                                int rows, int prj_dtype, hipStream_t st);
+                                int rows, int prj_dtype, int single_pass, hipStream_t st);
 hipError_t launch_batch_update(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
                                int rows, int waves, int potential, int prj_dtype,
-                               hipStream_t st);
+                               int single_pass, hipStream_t st);
 
 // Warm-start network (cnn.hip). Channels-last BF16 views (pointer, channel
 // stride, channel offset).

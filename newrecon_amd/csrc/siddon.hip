@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256) void project_4r_kernel(DevGeom g,
 
 // Batched projection + h (kind 3) over every case of the table in one launch;
 // schedule as batch_update_kernel.
-template <typename TI, typename TO, int TJ>
+template <typename TI, typename TO, int TJ, bool LOOP>
 __global__ __launch_bounds__(64 * TJ) void batch_project_kernel(
     const BatchCase* __restrict__ cases, const int* __restrict__ sched, int n_cases) {
     BatchWalk walk(sched, n_cases);
@@ -430,10 +430,11 @@ __global__ __launch_bounds__(64 * TJ) void batch_project_kernel(
         int slot, i, j;
         const bool in = bin_of_tile<TJ>(g, n_slots, t, slot, i, j);
         const ViewTab v = uniform_copy(bc.v[slot]);  // slot is uniform over the group
-        if (!in) continue;
-        project_bin<3, TI, TO>(g, v, slot, i, j, 1, (const TI*)uniform_ptr(bc.x_in),
-                               (TO*)uniform_ptr(bc.h), uniform_ptr(bc.y), uniform_ptr(bc.b),
-                               uniform_ptr(bc.r));
+        if (in)
+            project_bin<3, TI, TO>(g, v, slot, i, j, 1, (const TI*)uniform_ptr(bc.x_in),
+                                   (TO*)uniform_ptr(bc.h), uniform_ptr(bc.y), uniform_ptr(bc.b),
+                                   uniform_ptr(bc.r));
+        if (!LOOP) break;
     }
 }
 
@@ -536,20 +537,24 @@ hipError_t launch_project(const DevGeom& g, const ViewTab* views, int a0, int a1
 namespace {
 template <typename TO, int TJ>
 hipError_t batch_fwd_t(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
-                       hipStream_t st) {
-    hipLaunchKernelGGL((batch_project_kernel<float, TO, TJ>), dim3(grid), dim3(64 * TJ), 0, st,
-                       cases, sched, n_cases);
+                       int single_pass, hipStream_t st) {
+    if (single_pass)
+        hipLaunchKernelGGL((batch_project_kernel<float, TO, TJ, false>), dim3(grid),
+                           dim3(64 * TJ), 0, st, cases, sched, n_cases);
+    else
+        hipLaunchKernelGGL((batch_project_kernel<float, TO, TJ, true>), dim3(grid),
+                           dim3(64 * TJ), 0, st, cases, sched, n_cases);
     return hipGetLastError();
 }
 
 template <typename TO>
 hipError_t batch_fwd_o(int rows, const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
-                       hipStream_t st) {
+                       int single, hipStream_t st) {
     switch (rows) {
-        case 1: return batch_fwd_t<TO, 1>(cases, sched, n_cases, total, grid, st);
-        case 2: return batch_fwd_t<TO, 2>(cases, sched, n_cases, total, grid, st);
-        case 4: return batch_fwd_t<TO, 4>(cases, sched, n_cases, total, grid, st);
-        case 8: return batch_fwd_t<TO, 8>(cases, sched, n_cases, total, grid, st);
+        case 1: return batch_fwd_t<TO, 1>(cases, sched, n_cases, total, grid, single, st);
+        case 2: return batch_fwd_t<TO, 2>(cases, sched, n_cases, total, grid, single, st);
+        case 4: return batch_fwd_t<TO, 4>(cases, sched, n_cases, total, grid, single, st);
+        case 8: return batch_fwd_t<TO, 8>(cases, sched, n_cases, total, grid, single, st);
     }
     return hipErrorInvalidValue;
 }
@@ -560,12 +565,12 @@ int64_t batch_fwd_tiles(const DevGeom& g, int n_slots, int rows) {
 }
 
 hipError_t launch_batch_project(const BatchCase* cases, const int* sched, int n_cases, int total, int grid,
-                                int rows, int prj_dtype, hipStream_t st) {
+                                int rows, int prj_dtype, int single_pass, hipStream_t st) {
     if (total == 0) return hipSuccess;
     switch (prj_dtype) {
-        case OSTR_F32: return batch_fwd_o<float>(rows, cases, sched, n_cases, total, grid, st);
-        case OSTR_BF16: return batch_fwd_o<bf16_t>(rows, cases, sched, n_cases, total, grid, st);
-        case OSTR_F16: return batch_fwd_o<_Float16>(rows, cases, sched, n_cases, total, grid, st);
+        case OSTR_F32: return batch_fwd_o<float>(rows, cases, sched, n_cases, total, grid, single_pass, st);
+        case OSTR_BF16: return batch_fwd_o<bf16_t>(rows, cases, sched, n_cases, total, grid, single_pass, st);
+        case OSTR_F16: return batch_fwd_o<_Float16>(rows, cases, sched, n_cases, total, grid, single_pass, st);
     }
     return hipErrorInvalidValue;
 }
