@@ -131,7 +131,7 @@ def test_ostr_initialisers(rdbt_case, dev, init):
 def test_fbp_matches_oracle(rdbt_case, dev, window):
     """FDK-style filtering (cosine weight + windowed ramp along detector x) and
     the FBP image against the FP64 restatement (oracle/fbp.py): the filtered
-    views <= 1e-5 of their peak; the image, a sum of filtered samples of both
+    views <= 1e-5 of their peak and bit-identical on a repeat; the image, a sum of filtered samples of both
     signs that inherits their error, <= 3e-5 of the backprojection of their
     magnitudes (observed 4e-6 to 8e-6)."""
     from newrecon_amd.fbp import fbp_dbt, filter_taps
@@ -144,6 +144,10 @@ def test_fbp_matches_oracle(rdbt_case, dev, window):
     qr = F.fbp_filter(s.geom, s.angles, p, taps.astype(np.float32).astype(np.float64))
     err = float(np.abs(q.cpu().numpy() - qr).max() / np.abs(qr).max())
     assert err <= 1e-5, err
+    # the filter's GEMM is deterministic: a second call gives the same bits
+    q2 = S.fbp_filter(torch.from_numpy(p).to(dev),
+                      torch.from_numpy(taps.astype(np.float32)).to(dev), 0, s.n_views)
+    assert torch.equal(q, q2)
     import math
     from oracle import oracle as O
     scale = math.pi / (s.n_views * s.geom.dz)
