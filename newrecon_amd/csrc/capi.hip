@@ -140,9 +140,14 @@ int validate_geom(const ostr_geom* g) {
         return fail(nullptr, OSTR_E_ARG, "ostr_create: voxel and bin sizes must be positive");
     if ((int64_t)g->nbx * g->nby > (int64_t)1 << 30)
         return fail(nullptr, OSTR_E_ARG, "ostr_create: detector too large");
-    // 32-bit voxel offsets and buffer-resource sizes: < 2^30 voxels
-    if ((int64_t)g->nx * g->ny * g->nz >= (int64_t)1 << 30)
-        return fail(nullptr, OSTR_E_ARG, "ostr_create: volume must have < 2^30 voxels");
+    // 32-bit byte offsets and buffer-resource sizes: < 2^30 voxels, with a
+    // margin of two slices and two rows (the walk's out-of-volume indices
+    // one or two planes past an edge must not wrap into the volume's range)
+    const int64_t nxy = (int64_t)g->nx * g->ny;
+    if (nxy * g->nz + 2 * nxy + 2 * (int64_t)g->nx + 2 > (int64_t)1 << 30)
+        return fail(nullptr, OSTR_E_ARG,
+                    "ostr_create: volume too large (nx ny nz + 2 nx ny + 2 nx + 2 must be "
+                    "<= 2^30)");
     return OSTR_OK;
 }
 

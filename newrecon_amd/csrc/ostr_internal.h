@@ -170,21 +170,22 @@ struct VolReader {
         rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * sizeof(T)),
                                                  0x00020000);
     }
-    __device__ __forceinline__ float operator()(uint32_t i) const;
+    __device__ __forceinline__ float operator()(uint32_t i) const { return at_byte(i * sizeof(T)); }
+    // the element at byte offset b (a multiple of sizeof(T))
+    __device__ __forceinline__ float at_byte(uint32_t b) const;
 };
 template <>
-__device__ __forceinline__ float VolReader<float>::operator()(uint32_t i) const {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, i * 4u, 0, 0));
+__device__ __forceinline__ float VolReader<float>::at_byte(uint32_t b) const {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsrc, b, 0, 0));
 }
 template <>
-__device__ __forceinline__ float VolReader<bf16_t>::operator()(uint32_t i) const {
-    return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc, i * 2u, 0, 0)
-                           << 16);
+__device__ __forceinline__ float VolReader<bf16_t>::at_byte(uint32_t b) const {
+    return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rsrc, b, 0, 0) << 16);
 }
 template <>
-__device__ __forceinline__ float VolReader<_Float16>::operator()(uint32_t i) const {
-    unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(rsrc, i * 2u, 0, 0);
-    return (float)__builtin_bit_cast(_Float16, b);
+__device__ __forceinline__ float VolReader<_Float16>::at_byte(uint32_t b) const {
+    unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(rsrc, b, 0, 0);
+    return (float)__builtin_bit_cast(_Float16, h);
 }
 
 // Volume stores through a raw buffer resource (vector buffer stores, 32-bit

@@ -134,33 +134,36 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
     axis_entry(Z, lmin, sz, ez, g.z_p0, g.dz, g.nz, g.hp_z, g.last_z);
 
     const unsigned nx = g.nx, ny = g.ny, nz = g.nz;
-    // 32-bit voxel offset (the volume is limited to < 2^30 voxels)
-    uint32_t vid = ((uint32_t)Z.p * ny + (uint32_t)Y.p) * nx + (uint32_t)X.p;
-    const uint32_t sxs = (uint32_t)X.step, sys = (uint32_t)(Y.step * (int)nx),
-                   szs = (uint32_t)(Z.step * (int)(nx * ny));
+    // Each axis index is kept as its byte offset in the volume, px = X.p E,
+    // py = Y.p nx E, pz = Z.p nx ny E (E = element size), so one select per
+    // crossing updates both the in-volume test and the voxel's byte offset
+    // vid = px + py + pz. An index of -1 or -2 wraps to 2^32 - k stride, and
+    // the volume size is limited (at handle creation) to
+    // (nvox + 2 nx ny + 2 nx + 2) E <= 2^32, so it still compares as outside.
+    constexpr uint32_t E = sizeof(TI);
+    const uint32_t sxs = (uint32_t)X.step * E, sys = (uint32_t)Y.step * (nx * E),
+                   szs = (uint32_t)Z.step * (nx * ny * E);
+    const uint32_t bx = nx * E, by = ny * (nx * E), bz = nz * (ny * nx * E);
+    uint32_t px = (uint32_t)X.p * E, py = (uint32_t)Y.p * (nx * E),
+             pz = (uint32_t)Z.p * (nx * ny * E);
     float sum = 0.0f;
     float cur = lmin;
     // min of the three next crossings; the reference's cascade of `>` tests
     // picks the same value (no NaNs reach here)
     float at = fminf(fminf(X.next, Y.next), Z.next);
-    auto inside = [&]() {
-        return (unsigned)X.p < nx && (unsigned)Y.p < ny && (unsigned)Z.p < nz;
-    };
+    auto inside = [&]() { return px < bx && py < by && pz < bz; };
     auto advance = [&]() {
         if (at == X.next) {
-            X.p += X.step;
+            px += sxs;
             X.next = X.next + X.inc;
-            vid += sxs;
         }
         if (at == Y.next) {
-            Y.p += Y.step;
+            py += sys;
             Y.next = Y.next + Y.inc;
-            vid += sys;
         }
         if (at == Z.next) {
-            Z.p += Z.step;
+            pz += szs;
             Z.next = Z.next + Z.inc;
-            vid += szs;
         }
         cur = at;
         at = fminf(fminf(X.next, Y.next), Z.next);
@@ -184,17 +187,16 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
                 const bool act = active;
                 seg[k] = (at - cur) * len;
                 ok[k] = act && inside();
-                off[k] = ok[k] ? vid : 0u;
+                off[k] = ok[k] ? px + py + pz : 0u;
                 const bool sx = act && at == X.next;
                 const bool sy = act && at == Y.next;
                 const bool sz = act && at == Z.next;
-                X.p = sx ? X.p + X.step : X.p;
-                Y.p = sy ? Y.p + Y.step : Y.p;
-                Z.p = sz ? Z.p + Z.step : Z.p;
+                px += sx ? sxs : 0u;
+                py += sy ? sys : 0u;
+                pz += sz ? szs : 0u;
                 X.next = sx ? X.next + X.inc : X.next;
                 Y.next = sy ? Y.next + Y.inc : Y.next;
                 Z.next = sz ? Z.next + Z.inc : Z.next;
-                vid += (sx ? sxs : 0u) + (sy ? sys : 0u) + (sz ? szs : 0u);
                 const float nat = fminf(fminf(X.next, Y.next), Z.next);
                 cur = act ? at : cur;
                 at = act ? nat : at;
@@ -202,7 +204,7 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
             }
             float v[WALK_BATCH];
 #pragma unroll
-            for (int k = 0; k < WALK_BATCH; ++k) v[k] = ONES ? 1.0f : img(off[k]);
+            for (int k = 0; k < WALK_BATCH; ++k) v[k] = ONES ? 1.0f : img.at_byte(off[k]);
 #pragma unroll
             for (int k = 0; k < WALK_BATCH; ++k) {
                 sum = ok[k] ? fmaf(v[k], seg[k], sum) : sum;
@@ -215,7 +217,7 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
         while (keep_walking(lmax, at)) {
             float seg = (at - cur) * len;
             if (inside()) {
-                float v = ONES ? 1.0f : img(vid);
+                float v = ONES ? 1.0f : img.at_byte(px + py + pz);
                 sum = fmaf(v, seg, sum);
                 if (NORM) *norm = fmaf(1.0f, seg, *norm);
                 if (COUNT && seg > 0.0f) ++nseg;
@@ -225,7 +227,7 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
     }
     float seg = (lmax - cur) * len;
     if (inside()) {
-        float v = ONES ? 1.0f : img(vid);
+        float v = ONES ? 1.0f : img.at_byte(px + py + pz);
         sum = fmaf(v, seg, sum);
         if (NORM) *norm = fmaf(1.0f, seg, *norm);
         if (COUNT && seg > 0.0f) ++nseg;
