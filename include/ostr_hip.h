@@ -70,7 +70,9 @@ typedef struct ostr_handle ostr_handle;
  * float32 angles in radians (the reference's angle file, Tomo_OSTR_gold.cpp:
  * 1474-1488); `index` the initial slot->view table (NULL = identity). Per-view
  * sin/cos are evaluated once on the host in double and rounded to float.
- * Replaces the geometry upload of main() (Tomo_OSTR.cu:2653-2690). */
+ * Replaces the geometry upload of main() (Tomo_OSTR.cu:2653-2690). Limits
+ * (OSTR_E_ARG otherwise): volume dimensions < 32768, nx ny nz + 2 nx ny + 2 nx + 2
+ * <= 2^30 (32-bit byte offsets into the volume), nbx nby <= 2^30. */
 int ostr_create(const ostr_geom* geom, const float* angles_rad, const int32_t* index,
                 int32_t n_views, ostr_handle** out);
 void ostr_destroy(ostr_handle* h);

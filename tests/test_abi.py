@@ -55,3 +55,24 @@ def test_error_path_without_device():
     rc = lib.ostr_create(None, None, None, 0, ctypes.byref(h))
     assert rc == -1 and h.value is None
     assert b"geometry" in lib.ostr_last_error(None)
+
+
+def test_volume_limit_is_checked_before_any_device_call():
+    """nx ny nz + 2 nx ny + 2 nx + 2 must stay <= 2^30 (32-bit byte offsets
+    with a two-plane margin for the walk's out-of-volume indices)."""
+    if not os.path.exists(LIB):
+        pytest.skip("libostr_hip.so not built")
+    import numpy as np
+    from newrecon_amd.geometry import Geometry
+    lib = ctypes.CDLL(LIB)
+    lib.ostr_create.restype = ctypes.c_int
+    lib.ostr_last_error.restype = ctypes.c_char_p
+    lib.ostr_last_error.argtypes = [ctypes.c_void_p]
+    angles = np.zeros(1, np.float32)
+    # 32767 x 32767 x 1 voxels: below 2^30, but not with the two-plane margin
+    g = Geometry.half_cone(32767, 32767, 1, 0.1, 0.1, 1.0, 256, 128, 0.5, 0.5)
+    h = ctypes.c_void_p()
+    rc = lib.ostr_create(ctypes.byref(g.to_c()), angles.ctypes.data_as(ctypes.c_void_p), None,
+                         1, ctypes.byref(h))
+    assert rc == -1 and h.value is None
+    assert b"volume too large" in lib.ostr_last_error(None)
