@@ -159,7 +159,15 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
                 const float dw = num * __builtin_amdgcn_rcpf(den);
                 const uint32_t off = base + (uint32_t)jl * nbx;
                 float p00, p10, p01, p11;
-                if (wave_i_ok && j0 && j1) {
+                // both columns and rows on the detector: the two taps of a
+                // row as one pair (one load for 16-bit h; the pair of row
+                // jl + 1 may read one element past its end, which exists
+                // unless that row is the last of the last slot)
+                if (wave_i_ok && j0 && j1 &&
+                    (sizeof(TP) == 4 || a + 1 < n || jl + 2 < g.nby)) {
+                    P.pair(off, p00, p10);
+                    P.pair(off + nbx, p01, p11);
+                } else if (wave_i_ok && j0 && j1) {
                     p00 = P(off);
                     p10 = P(off + 1);
                     p01 = P(off + nbx);

@@ -173,6 +173,10 @@ struct VolReader {
     __device__ __forceinline__ float operator()(uint32_t i) const { return at_byte(i * sizeof(T)); }
     // the element at byte offset b (a multiple of sizeof(T))
     __device__ __forceinline__ float at_byte(uint32_t b) const;
+    // elements e and e + 1; 2-byte types read them with one 8-byte load from
+    // the 4-byte boundary at or below e, so when e is odd element e + 2 must
+    // lie inside the buffer (the caller guarantees it)
+    __device__ __forceinline__ void pair(uint32_t e, float& a, float& b) const;
 };
 template <>
 __device__ __forceinline__ float VolReader<float>::at_byte(uint32_t b) const {
@@ -186,6 +190,30 @@ template <>
 __device__ __forceinline__ float VolReader<_Float16>::at_byte(uint32_t b) const {
     unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(rsrc, b, 0, 0);
     return (float)__builtin_bit_cast(_Float16, h);
+}
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+// the 32 bits holding elements e (low half) and e + 1 (high half) of a
+// 2-byte array, from one aligned 8-byte load and a funnel shift
+__device__ __forceinline__ uint32_t pair16(__amdgpu_buffer_rsrc_t rsrc, uint32_t e) {
+    const u32x2_t w = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (e & ~1u) * 2u, 0, 0);
+    return __builtin_amdgcn_alignbit(w.y, w.x, (e & 1u) * 16u);
+}
+template <>
+__device__ __forceinline__ void VolReader<float>::pair(uint32_t e, float& a, float& b) const {
+    a = at_byte(e * 4u);
+    b = at_byte(e * 4u + 4u);
+}
+template <>
+__device__ __forceinline__ void VolReader<bf16_t>::pair(uint32_t e, float& a, float& b) const {
+    const uint32_t p = pair16(rsrc, e);
+    a = __uint_as_float(p << 16);
+    b = __uint_as_float(p & 0xFFFF0000u);
+}
+template <>
+__device__ __forceinline__ void VolReader<_Float16>::pair(uint32_t e, float& a, float& b) const {
+    const uint32_t p = pair16(rsrc, e);
+    a = (float)__builtin_bit_cast(_Float16, (unsigned short)(p & 0xFFFFu));
+    b = (float)__builtin_bit_cast(_Float16, (unsigned short)(p >> 16));
 }
 
 // Volume stores through a raw buffer resource (vector buffer stores, 32-bit
