@@ -173,9 +173,9 @@ struct VolReader {
     __device__ __forceinline__ float operator()(uint32_t i) const { return at_byte(i * sizeof(T)); }
     // the element at byte offset b (a multiple of sizeof(T))
     __device__ __forceinline__ float at_byte(uint32_t b) const;
-    // elements e and e + 1; 2-byte types read them with one 8-byte load from
-    // the 4-byte boundary at or below e, so when e is odd element e + 2 must
-    // lie inside the buffer (the caller guarantees it)
+    // elements e and e + 1 from one 8-byte load: for 4-byte types at e, for
+    // 2-byte types from the 4-byte boundary at or below e, so when e is odd
+    // element e + 2 must lie inside the buffer (the caller guarantees it)
     __device__ __forceinline__ void pair(uint32_t e, float& a, float& b) const;
 };
 template <>
@@ -200,8 +200,9 @@ __device__ __forceinline__ uint32_t pair16(__amdgpu_buffer_rsrc_t rsrc, uint32_t
 }
 template <>
 __device__ __forceinline__ void VolReader<float>::pair(uint32_t e, float& a, float& b) const {
-    a = at_byte(e * 4u);
-    b = at_byte(e * 4u + 4u);
+    const u32x2_t w = __builtin_amdgcn_raw_buffer_load_b64(rsrc, e * 4u, 0, 0);
+    a = __uint_as_float(w.x);
+    b = __uint_as_float(w.y);
 }
 template <>
 __device__ __forceinline__ void VolReader<bf16_t>::pair(uint32_t e, float& a, float& b) const {
