@@ -718,7 +718,7 @@ OSTR_API int ostr_momentum_step(ostr_handle* h, float* x_next, const float* z,
     int rc;
     if ((rc = use_device(h))) return rc;
     hipStream_t st = (hipStream_t)stream;
-    if ((rc = ensure_dwork(h, 1024 + 1, st, "ostr_momentum_step"))) return rc;
+    if ((rc = ensure_dwork(h, 1024 + 4, st, "ostr_momentum_step"))) return rc;
     int64_t n = (int64_t)h->geom.nx * h->geom.ny * h->geom.nz;
     hipError_t e = ostr::launch_momentum_step(x_next, z, z_prev, x_k, n, h->dwork, state, st);
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_momentum_step");

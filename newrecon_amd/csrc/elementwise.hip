@@ -416,10 +416,15 @@ __global__ __launch_bounds__(256) void diff_norms_kernel(const T* __restrict__ a
 
 // partials[k] = sum over block k's strided elements of (x - z)(z - zp), FP64
 // (the gradient-restart test of accelerated SQS; fixed grid: deterministic).
+// With state: block 0 also copies the 4 state doubles to snap, which the
+// momentum step's second kernel reads while it rewrites state.
 __global__ __launch_bounds__(256) void restart_dot_kernel(const float* __restrict__ x,
                                                           const float* __restrict__ z,
                                                           const float* __restrict__ zp, int64_t n,
-                                                          double* __restrict__ partials) {
+                                                          double* __restrict__ partials,
+                                                          const double* __restrict__ state,
+                                                          double* __restrict__ snap) {
+    if (state && blockIdx.x == 0 && threadIdx.x < 4) snap[threadIdx.x] = state[threadIdx.x];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
          i += (int64_t)NORM_BLOCKS * 256) {
@@ -474,13 +479,44 @@ __global__ __launch_bounds__(256) void momentum_coef_kernel(const double* __rest
     }
 }
 
-// extrapolate_kernel with c = (float) state[1] read on the device.
-__global__ __launch_bounds__(256) void extrapolate_state_kernel(float* __restrict__ x,
-                                                                const float* __restrict__ z,
-                                                                const float* __restrict__ zp,
-                                                                const double* __restrict__ state,
-                                                                int64_t n) {
-    const float c = (float)state[1];
+// The momentum step after restart_dot_kernel, in one launch: every block sums
+// the n_p partials in the fixed order of momentum_coef_kernel (so each gets
+// the same dot) and applies the coefficient rule to the state snapshot snap
+// (t, c, restarts, last test value, taken by restart_dot_kernel); block 0
+// writes the new state. Then x = max(0, z + c (z - zp)) with c = (float) of
+// the new coefficient, as extrapolate_kernel. The grid is at most
+// MOMENTUM_BLOCKS blocks, which bounds the repeated partial sums.
+constexpr int MOMENTUM_BLOCKS = 512;
+
+__global__ __launch_bounds__(256) void momentum_extrapolate_kernel(
+    float* __restrict__ x, const float* __restrict__ z, const float* __restrict__ zp,
+    const double* __restrict__ p, int64_t n_p, const double* __restrict__ snap,
+    double* __restrict__ state, int64_t n) {
+    __shared__ float c_blk;
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n_p; i += 256) acc += p[i];
+    const double dot = block_sum_256(acc);
+    if (threadIdx.x == 0) {
+        const double t = snap[0];
+        double tn, c, restarts = snap[2];
+        if (dot > 0.0) {
+            tn = 1.0;
+            c = 0.0;
+            restarts = restarts + 1.0;
+        } else {
+            tn = 0.5 * (1.0 + sqrt(1.0 + 4.0 * t * t));
+            c = (t - 1.0) / tn;
+        }
+        c_blk = (float)c;
+        if (blockIdx.x == 0) {
+            state[0] = tn;
+            state[1] = c;
+            state[2] = restarts;
+            state[3] = dot;
+        }
+    }
+    __syncthreads();
+    const float c = c_blk;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const float v = fmaf(c, z[i] - zp[i], z[i]);
@@ -498,16 +534,20 @@ hipError_t launch_momentum_init(double* state, hipStream_t st) {
 hipError_t launch_momentum_step(float* x_next, const float* z, const float* z_prev,
                                 const float* x_k, int64_t n, double* partials, double* state,
                                 hipStream_t st) {
+    double* snap = partials + NORM_BLOCKS;
     hipLaunchKernelGGL(restart_dot_kernel, dim3(NORM_BLOCKS), dim3(256), 0, st, x_k, z, z_prev,
-                       n, partials);
+                       n, partials, (const double*)state, snap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(momentum_coef_kernel, dim3(1), dim3(256), 0, st, partials,
-                       (int64_t)NORM_BLOCKS, state);
-    e = hipGetLastError();
-    if (e != hipSuccess || n <= 0) return e;
-    hipLaunchKernelGGL(extrapolate_state_kernel, dim3(grid_for(n)), dim3(256), 0, st, x_next, z,
-                       z_prev, state, n);
+    if (n <= 0) {
+        hipLaunchKernelGGL(momentum_coef_kernel, dim3(1), dim3(256), 0, st, partials,
+                           (int64_t)NORM_BLOCKS, state);
+        return hipGetLastError();
+    }
+    const unsigned blocks = grid_for(n) < MOMENTUM_BLOCKS ? grid_for(n) : MOMENTUM_BLOCKS;
+    hipLaunchKernelGGL(momentum_extrapolate_kernel, dim3(blocks), dim3(256), 0, st, x_next, z,
+                       z_prev, (const double*)partials, (int64_t)NORM_BLOCKS,
+                       (const double*)snap, state, n);
     return hipGetLastError();
 }
 
@@ -538,7 +578,7 @@ hipError_t launch_diff_norms(const void* a, const void* b, int dtype, int64_t n,
 hipError_t launch_restart_dot(const float* x, const float* z, const float* zp, int64_t n,
                               double* partials, double* out, hipStream_t st) {
     hipLaunchKernelGGL(restart_dot_kernel, dim3(NORM_BLOCKS), dim3(256), 0, st, x, z, zp, n,
-                       partials);
+                       partials, (const double*)nullptr, (double*)nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_reduce_partials(partials, NORM_BLOCKS, out, st);

@@ -353,7 +353,8 @@ hipError_t launch_sart_update(const DevGeom& g, const ViewTab* views, int a0, in
 hipError_t launch_huber_prior(const DevGeom& g, float* prior, const float* img, float delta,
                               hipStream_t st);
 // Accelerated SQS momentum with the restart decision on the device (no host
-// round trip): state = {t, c, restarts, last test value}.
+// round trip): state = {t, c, restarts, last test value}; partials: 1024 + 4
+// doubles (the restart-test partial sums, then a snapshot of state).
 hipError_t launch_momentum_init(double* state, hipStream_t st);
 hipError_t launch_momentum_step(float* x_next, const float* z, const float* z_prev,
                                 const float* x_k, int64_t n, double* partials, double* state,
