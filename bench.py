@@ -43,12 +43,26 @@ PROFILED_KERNEL = {"fproj_h": "project_kernel<3, float, float>",
                    "update_fused": "update_fused_kernel<2, float, float>"}
 
 
+VALU_JSON = os.path.join(ROOT, "profiles",
+                         "valu_fproj_h-update_fused_dbt-clinical-1400x600x60-15v.json")
+
+
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (clinical workload), or None."""
     try:
         data = json.load(open(PROFILE_JSON))
         return data["kernels"][PROFILED_KERNEL[kernel]]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def profiled_valu(kernel):
+    """(VALU issue share, VALU busy) of `kernel` from the committed rocprofv3
+    SQ-counter summary (clinical workload), or None."""
+    try:
+        k = json.load(open(VALU_JSON))["kernels"][PROFILED_KERNEL[kernel]]
+        return round(k["valu_issue"], 3), round(k["valu_busy"], 3)
     except (OSError, KeyError, ValueError):
         return None
 
@@ -68,21 +82,44 @@ def cpu_baseline(scan, x_host, y_host, b_host, threads):
     b_sub = np.ascontiguousarray(b_host[views])
     d = np.ones(g.vol_shape, np.float32)  # precompute is not part of an iteration
     O.lib()
-    t0 = time.perf_counter()
-    l = O.fproj(g, scan.angles, x_host, 0, vs, index=idx, nthreads=threads)
-    h = O.compute_h(y_sub, b_sub, l)
-    capl = O.bproj_sbp(g, scan.angles, h, 0, vs, index=idx, nthreads=threads)
-    rdd, rd = O.prior(g, x_host, 2, scan.recon.delta, nthreads=threads)
-    O.update_est(x_host, capl, rd, d, rdd, scan.recon.beta, float(M))
-    dt = time.perf_counter() - t0
-    return {
+
+    def subset_step(nt):
+        t0 = time.perf_counter()
+        l = O.fproj(g, scan.angles, x_host, 0, vs, index=idx, nthreads=nt)
+        h = O.compute_h(y_sub, b_sub, l)
+        capl = O.bproj_sbp(g, scan.angles, h, 0, vs, index=idx, nthreads=nt)
+        rdd, rd = O.prior(g, x_host, 2, scan.recon.delta, nthreads=nt)
+        O.update_est(x_host, capl, rd, d, rdd, scan.recon.beta, float(M))
+        return time.perf_counter() - t0
+
+    dt = subset_step(threads)
+    out = {
         "value": round(dt * M, 3),
         "unit": "s/iter",
         "cores": threads,
         "kind": "port",
         "sample": f"1 of {M} subsets ({vs} views) of one iteration of the same workload, "
                   f"{dt:.2f} s measured, scaled x{M}",
+        "cpu_model": cpu_model(),
     }
+    if threads > 1:
+        d1 = subset_step(1)
+        out["single_thread"] = {"value": round(d1 * M, 3), "unit": "s/iter", "cores": 1,
+                                "sample": f"the same subset on 1 thread, {d1:.2f} s, "
+                                          f"scaled x{M}"}
+    return out
+
+
+def cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def bench_batch(args, dev):
@@ -594,6 +631,13 @@ def main(argv=None):
                         if args.config == "clinical" and profiled_traffic(dominant) else None),
             "traffic_source": os.path.relpath(PROFILE_JSON, ROOT),
             "algorithmic_bytes": bytes_dom,
+            # what bounds the kernel instead: the share of the VALU issue
+            # slots it uses and its VALU busy fraction (SQ counters)
+            "valu_issue": (profiled_valu(dominant) or (None, None))[0]
+                          if args.config == "clinical" else None,
+            "valu_busy": (profiled_valu(dominant) or (None, None))[1]
+                         if args.config == "clinical" else None,
+            "valu_source": os.path.relpath(VALU_JSON, ROOT),
         },
         "cpu_baseline": cpu,
         "time_to_1e-3": ttt,
