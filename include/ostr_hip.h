@@ -72,7 +72,13 @@ typedef struct ostr_handle ostr_handle;
  * sin/cos are evaluated once on the host in double and rounded to float.
  * Replaces the geometry upload of main() (Tomo_OSTR.cu:2653-2690). Limits
  * (OSTR_E_ARG otherwise): volume dimensions < 32768, nx ny nz + 2 nx ny + 2 nx + 2
- * <= 2^30 (32-bit byte offsets into the volume), nbx nby <= 2^30. */
+ * <= 2^30 (32-bit byte offsets into the volume), nbx nby <= 2^30. The
+ * backprojecting calls (SBP, exact transpose, OS-SART and fused updates, the d
+ * precompute, BP image) also need min(views, 64) nbx nby elem_bytes < 2^32 - 16
+ * per call (32-bit byte offsets into the sinogram; OSTR_E_RANGE otherwise). On
+ * a handle used under stream capture, work buffers that later calls outgrow are
+ * kept until ostr_destroy (a recorded graph may still write to them), and a
+ * call that would have to grow one while capturing fails with OSTR_E_ARG. */
 int ostr_create(const ostr_geom* geom, const float* angles_rad, const int32_t* index,
                 int32_t n_views, ostr_handle** out);
 void ostr_destroy(ostr_handle* h);

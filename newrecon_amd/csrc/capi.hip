@@ -31,6 +31,11 @@ struct ostr_handle {
     size_t work_bytes = 0;
     double* dwork = nullptr;  // FP64 partial sums of the reductions (+2 results)
     size_t dwork_elems = 0;
+    // Set once a call on this handle ran on a capturing stream: a graph then
+    // holds work/dwork addresses and may be replayed after any later call, so
+    // an outgrown buffer is kept in `retired` (freed by ostr_destroy).
+    bool captured = false;
+    std::vector<void*> retired;
     std::string err;
 };
 
@@ -79,6 +84,20 @@ int check_dtype(ostr_handle* h, int dt, const char* fn) {
     return OSTR_OK;
 }
 
+// The backprojectors read a launch's sinogram slots (at most MAX_SLOTS per
+// launch) through a buffer resource with 32-bit byte offsets: one launch's
+// span must stay below 2^32 bytes.
+int check_sino_span(ostr_handle* h, int64_t n_slots, int64_t elem_bytes, const char* fn) {
+    const int64_t slots = n_slots < ostr::MAX_SLOTS ? n_slots : ostr::MAX_SLOTS;
+    const int64_t bytes = slots * h->geom.nbx * h->geom.nby * elem_bytes;
+    if (bytes > (int64_t)0xFFFFFFF0)
+        return fail(h, OSTR_E_RANGE,
+                    "%s: %lld views of %dx%d bins span %lld bytes per launch; at most 2^32 - 16 "
+                    "(use fewer views per call)",
+                    fn, (long long)slots, h->geom.nbx, h->geom.nby, (long long)bytes);
+    return OSTR_OK;
+}
+
 ostr::ViewTab view_of(const ostr_geom& g, float theta) {
     ostr::ViewTab v;
     v.s = (float)std::sin((double)theta);
@@ -98,36 +117,57 @@ int upload_views(ostr_handle* h) {
     return OSTR_OK;
 }
 
-// Grow the handle's device work buffer to at least `bytes` (stream-ordered:
-// the old buffer may still be read by queued work on `st`).
-int ensure_work(ostr_handle* h, size_t bytes, hipStream_t st, const char* fn) {
-    if (h->work_bytes >= bytes) return OSTR_OK;
-    if (h->work) {
+// True when `st` is recording a graph. The legacy null stream cannot be
+// captured and is not queried (a query of it during another stream's global
+// capture would invalidate that capture).
+bool capturing(hipStream_t st) {
+    if (!st) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// Grow one of the handle's device work buffers to at least `bytes`
+// (stream-ordered: the old buffer may still be read by queued work on `st`).
+// After a capture the old buffer is retired, not freed: a recorded graph may
+// still write to it on replay. Growing during capture is an error (neither a
+// synchronize nor an allocation may be recorded).
+int grow(ostr_handle* h, void** buf, size_t* have, size_t bytes, hipStream_t st,
+         const char* fn) {
+    const bool cap = capturing(st);
+    h->captured = h->captured || cap;
+    if (*have >= bytes) return OSTR_OK;
+    if (cap)
+        return fail(h, OSTR_E_ARG,
+                    "%s: work buffer must grow to %zu bytes during stream capture; make the "
+                    "same call once before capturing",
+                    fn, bytes);
+    if (*buf) {
         hipError_t e = hipStreamSynchronize(st);
         if (e != hipSuccess) return hip_fail(h, e, fn);
-        (void)hipFree(h->work);
-        h->work = nullptr;
-        h->work_bytes = 0;
+        if (h->captured)
+            h->retired.push_back(*buf);
+        else
+            (void)hipFree(*buf);
+        *buf = nullptr;
+        *have = 0;
     }
-    hipError_t e = hipMalloc(&h->work, bytes);
+    hipError_t e = hipMalloc(buf, bytes);
     if (e != hipSuccess) return hip_fail(h, e, fn);
-    h->work_bytes = bytes;
+    *have = bytes;
     return OSTR_OK;
 }
 
+int ensure_work(ostr_handle* h, size_t bytes, hipStream_t st, const char* fn) {
+    return grow(h, &h->work, &h->work_bytes, bytes, st, fn);
+}
+
 int ensure_dwork(ostr_handle* h, size_t elems, hipStream_t st, const char* fn) {
-    if (h->dwork_elems >= elems) return OSTR_OK;
-    if (h->dwork) {
-        hipError_t e = hipStreamSynchronize(st);
-        if (e != hipSuccess) return hip_fail(h, e, fn);
-        (void)hipFree(h->dwork);
-        h->dwork = nullptr;
-        h->dwork_elems = 0;
-    }
-    hipError_t e = hipMalloc(&h->dwork, elems * sizeof(double));
-    if (e != hipSuccess) return hip_fail(h, e, fn);
-    h->dwork_elems = elems;
-    return OSTR_OK;
+    size_t have = h->dwork_elems * sizeof(double);
+    void* buf = h->dwork;
+    int rc = grow(h, &buf, &have, elems * sizeof(double), st, fn);
+    h->dwork = (double*)buf;
+    h->dwork_elems = have / sizeof(double);
+    return rc;
 }
 
 int validate_geom(const ostr_geom* g) {
@@ -235,6 +275,7 @@ OSTR_API void ostr_destroy(ostr_handle* h) {
     ostr::fbp_blas_destroy(h->blas);
     if (h->work) (void)hipFree(h->work);
     if (h->dwork) (void)hipFree(h->dwork);
+    for (void* p : h->retired) (void)hipFree(p);
     delete h;
 }
 
@@ -306,6 +347,8 @@ OSTR_API int ostr_bproj_sbp(ostr_handle* h, float* img_accum, const void* prj,
         (rc = check_dtype(h, prj_dtype, "ostr_bproj_sbp")))
         return rc;
     if (!img_accum || !prj) return fail(h, OSTR_E_ARG, "ostr_bproj_sbp: NULL buffer");
+    if ((rc = check_sino_span(h, a1 - a0, ostr::elem_size(prj_dtype), "ostr_bproj_sbp")))
+        return rc;
     if ((rc = use_device(h))) return rc;
     if (a0 == a1) return OSTR_OK;
     hipStream_t st = (hipStream_t)stream;
@@ -355,6 +398,7 @@ int bproj_sart(ostr_handle* h, float* img, const float* diff, const float* prior
     if (!img || !diff) return fail(h, OSTR_E_ARG, "%s: NULL buffer", fn);
     if (a1 - a0 > ostr::MAX_SLOTS)
         return fail(h, OSTR_E_RANGE, "%s: at most %d views per call", fn, ostr::MAX_SLOTS);
+    if ((rc = check_sino_span(h, a1 - a0, 4, fn))) return rc;
     if ((rc = use_device(h))) return rc;
     if (a0 == a1) return OSTR_OK;
     hipStream_t st = (hipStream_t)stream;
@@ -514,6 +558,7 @@ OSTR_API int ostr_sart_update_fused(ostr_handle* h, const float* x_in, float* x_
     if (a1 - a0 > ostr::MAX_SLOTS)
         return fail(h, OSTR_E_RANGE, "ostr_sart_update_fused: at most %d views per call",
                     ostr::MAX_SLOTS);
+    if ((rc = check_sino_span(h, a1 - a0, 4, "ostr_sart_update_fused"))) return rc;
     if ((rc = use_device(h))) return rc;
     hipError_t e = ostr::launch_sart_update_fused(h->dg, h->views_host.data(), a0, a1, x_in,
                                                   x_out, diff_rows, lambda, beta, delta,
@@ -530,6 +575,8 @@ OSTR_API int ostr_bproj_matched(ostr_handle* h, float* img_accum, const void* pr
         (rc = check_dtype(h, prj_dtype, "ostr_bproj_matched")))
         return rc;
     if (!img_accum || !prj) return fail(h, OSTR_E_ARG, "ostr_bproj_matched: NULL buffer");
+    if ((rc = check_sino_span(h, a1 - a0, ostr::elem_size(prj_dtype), "ostr_bproj_matched")))
+        return rc;
     if ((rc = use_device(h))) return rc;
     if (a0 == a1) return OSTR_OK;
     hipStream_t st = (hipStream_t)stream;
@@ -555,6 +602,7 @@ OSTR_API int ostr_bp_init(ostr_handle* h, float* img, const float* p_or_y, const
     if (!img || !p_or_y || (counts && !b))
         return fail(h, OSTR_E_ARG, "ostr_bp_init: NULL buffer");
     int rc;
+    if ((rc = check_sino_span(h, 1, 4, "ostr_bp_init"))) return rc;
     if ((rc = use_device(h))) return rc;
     hipStream_t st = (hipStream_t)stream;
     const ostr_geom& G = h->geom;
@@ -597,6 +645,8 @@ OSTR_API int ostr_bproj_sbp_rows(ostr_handle* h, float* img_accum, const void* p
         (rc = check_dtype(h, prj_dtype, "ostr_bproj_sbp_rows")))
         return rc;
     if (!img_accum || !prj) return fail(h, OSTR_E_ARG, "ostr_bproj_sbp_rows: NULL buffer");
+    if ((rc = check_sino_span(h, a1 - a0, ostr::elem_size(prj_dtype), "ostr_bproj_sbp_rows")))
+        return rc;
     if ((rc = use_device(h))) return rc;
     hipError_t e = ostr::launch_sbp_rows(h->dg, h->views_host.data(), a0, a1, img_accum, prj,
                                          prj_dtype, (hipStream_t)stream);
@@ -771,6 +821,7 @@ OSTR_API int ostr_precompute_d(ostr_handle* h, float* d, const float* y, const f
     if (gamma_mode != OSTR_GAMMA_RAY_LENGTH && gamma_mode != OSTR_GAMMA_ROW_SUM)
         return fail(h, OSTR_E_ARG, "ostr_precompute_d: unknown gamma mode %d", gamma_mode);
     int rc;
+    if ((rc = check_sino_span(h, h->n_views, 4, "ostr_precompute_d"))) return rc;
     if ((rc = use_device(h))) return rc;
     hipStream_t st = (hipStream_t)stream;
     const int64_t nall = (int64_t)h->n_views * h->geom.nbx * h->geom.nby;
@@ -823,7 +874,11 @@ OSTR_API int ostr_update_fused(ostr_handle* h, const void* x_in, void* x_out, in
         return fail(h, OSTR_E_ARG, "ostr_update_fused: delta cannot be zero");
     if (!(z_xy_ratio > 0.0f))
         return fail(h, OSTR_E_ARG, "ostr_update_fused: z_xy_ratio must be > 0");
-    if (a1 - a0 > 64) return fail(h, OSTR_E_RANGE, "ostr_update_fused: at most 64 views per subset");
+    if (a1 - a0 > ostr::MAX_SLOTS)
+        return fail(h, OSTR_E_RANGE, "ostr_update_fused: at most %d views per subset",
+                    ostr::MAX_SLOTS);
+    if ((rc = check_sino_span(h, a1 - a0, ostr::elem_size(prj_dtype), "ostr_update_fused")))
+        return rc;
     if ((rc = use_device(h))) return rc;
     hipError_t e = ostr::launch_update_fused(h->dg, h->views_host.data(), a0, a1, x_in, x_out,
                                              img_dtype, d, hsub, prj_dtype, beta, capl_scale,
@@ -853,6 +908,7 @@ OSTR_API int ostr_update_fused_matched(ostr_handle* h, const float* x_in, float*
     if (a1 - a0 > ostr::MAX_SLOTS)
         return fail(h, OSTR_E_RANGE, "ostr_update_fused_matched: at most %d views per subset",
                     ostr::MAX_SLOTS);
+    if ((rc = check_sino_span(h, a1 - a0, 4, "ostr_update_fused_matched"))) return rc;
     if ((rc = use_device(h))) return rc;
     hipError_t e = ostr::launch_update_fused_matched(h->dg, h->views_host.data(), a0, a1, x_in,
                                                      x_out, d, hsub, beta, capl_scale, potential,
@@ -1037,6 +1093,10 @@ int batch_stage(ostr_batch* bt, const ostr_batch_case* cases, int n, int prj_dty
         if (kind == 1) {
             if (k.x_in == k.x_out)
                 return bfail(bt, OSTR_E_ARG, "%s: case %d: x_in and x_out alias", fn, c);
+            if ((int64_t)(k.a1 - k.a0) * k.h->geom.nbx * k.h->geom.nby *
+                    ostr::elem_size(prj_dtype) > (int64_t)0xFFFFFFF0)
+                return bfail(bt, OSTR_E_RANGE, "%s: case %d: the subset's h spans 2^32 bytes "
+                             "or more (32-bit offsets)", fn, c);
             if (potential != OSTR_PRIOR_Q2 && k.delta == 0.0f)
                 return bfail(bt, OSTR_E_ARG, "%s: case %d: delta cannot be zero", fn, c);
             if (!(k.z_xy_ratio > 0.0f))

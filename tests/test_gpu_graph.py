@@ -64,3 +64,34 @@ def test_device_momentum_matches_host_formula(small_case, dev):
     st = state.cpu().numpy()
     assert st[0] == 1.0 and st[1] == 0.0 and st[2] == 1.0 and st[3] > 0
     assert torch.equal(xn, z)
+
+
+@pytest.mark.parametrize("backprojector,M,momentum,fused", [("matched", 1, True, True),
+                                                            ("sbp", 3, False, False),
+                                                            ("matched", 3, False, False)])
+def test_graph_replay_after_work_buffers_grow(small_case, dev, backprojector, M, momentum, fused):
+    """Calls that grow the handle's work buffers between replays (the FP64
+    objective and norms, the BP image) must not free memory a recorded graph
+    still writes: the momentum step's partials and the unfused
+    backprojectors' transposes live there. Replays equal the eager run."""
+    s, x, y, b = small_case
+    cfg = OSTRConfig(n_subsets=M, beta=s.recon.beta, delta=s.recon.delta,
+                     backprojector=backprojector, momentum=momentum, fused=fused)
+    eager = OSTR(s.geom, s.angles, y, b, cfg=cfg, x0="bp", device=dev)
+    graph = OSTR(s.geom, s.angles, y, b, cfg=cfg, x0="bp", device=dev)
+    yd = torch.as_tensor(y, device=dev)
+    bd = torch.as_tensor(b, device=dev)
+    done = 0
+    for chunk in (6, 6, 7):
+        graph.iterate(chunk, graph=True)
+        # grow dwork (objective > norms > momentum partials) and work (BP image)
+        f = graph.objective()
+        n = graph.sys.diff_norms(graph.estimate, graph.estimate)
+        bp = graph.sys.bp_init(y=yd, b=bd)
+        torch.cuda.synchronize()
+        eager.iterate(chunk)
+        done += chunk
+        assert np.isfinite(f) and n[0] == 0.0 and torch.isfinite(bp).all()
+        assert graph.iteration == eager.iteration == done
+        assert torch.equal(graph.estimate, eager.estimate), chunk
+    assert graph._graph is not None
