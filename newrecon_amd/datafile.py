@@ -17,6 +17,7 @@ and reconstruction outputs.
 from __future__ import annotations
 
 import dataclasses
+import re
 import os
 from typing import Dict, List, Optional
 
@@ -75,6 +76,25 @@ def read_param(tokens: List[str], label: str, count: int = 1, numeric: bool = Tr
     return vals
 
 
+_INT_PREFIX = re.compile(r"\s*([+-]?\d+)")
+_FLOAT_PREFIX = re.compile(r"\s*([+-]?(?:(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?|inf(?:inity)?|nan))",
+                           re.IGNORECASE)
+
+
+def atoi(v: str) -> int:
+    """C atoi (readdata.cu:114): the leading integer of v ("512.0" -> 512),
+    0 when there is none."""
+    m = _INT_PREFIX.match(v)
+    return int(m.group(1)) if m else 0
+
+
+def atof(v: str) -> float:
+    """C atof (readdata.cu:300): the leading decimal number of v ("0.1mm" ->
+    0.1), 0.0 when there is none."""
+    m = _FLOAT_PREFIX.match(v)
+    return float(m.group(1)) if m else 0.0
+
+
 @dataclasses.dataclass
 class RunFile:
     geom: Geometry
@@ -89,11 +109,11 @@ def load_datafile(path: str) -> RunFile:
     tok = _tokens(path)
     g = {}
     for key, field in INT_KEYS.items():
-        g[field] = int(read_param(tok, key)[0])
+        g[field] = atoi(read_param(tok, key)[0])
     for key, field in FLOAT_KEYS.items():
-        g[field] = float(np.float32(read_param(tok, key)[0]))
-    run = {k: int(read_param(tok, k)[0]) for k in RUN_INT_KEYS}
-    run.update({k: float(np.float32(read_param(tok, k)[0])) for k in RUN_FLOAT_KEYS})
+        g[field] = float(np.float32(atof(read_param(tok, key)[0])))
+    run = {k: atoi(read_param(tok, k)[0]) for k in RUN_INT_KEYS}
+    run.update({k: float(np.float32(atof(read_param(tok, k)[0]))) for k in RUN_FLOAT_KEYS})
     paths = {k: read_param(tok, k, numeric=False)[0] for k in STRING_KEYS}
     geom = Geometry(**g)
     geom.validate()

@@ -78,6 +78,23 @@ def test_datafile_errors(tmp_path, edit, msg):
         load_datafile(os.path.join(tmp_path, "missing.dat"))
 
 
+def test_datafile_values_parse_like_atoi_atof(tmp_path):
+    """readdata.cu converts with atoi/atof (:114, :300): the leading number of
+    a value counts and the rest is ignored."""
+    from newrecon_amd.datafile import atof, atoi
+    assert atoi("512.0") == 512 and atoi("-3x") == -3 and atoi("+7") == 7 and atoi("-") == 0
+    assert atof("0.1mm") == 0.1 and atof("1e3,") == 1000.0 and atof(".5") == 0.5
+    assert atof("-2.5e-1z") == -0.25 and atof("+") == 0.0
+    path = os.path.join(tmp_path, "run.dat")
+    text = (DATAFILE.format(d=tmp_path).replace("image_size_x 200", "image_size_x 200.0")
+            .replace("Voxel_size_z_mm 1", "Voxel_size_z_mm 1mm")
+            .replace("Beta 1000", "Beta 1e3,"))
+    with open(path, "w") as f:
+        f.write(text)
+    run = load_datafile(path)
+    assert run.geom.nx == 200 and run.geom.dz == 1.0 and run.cfg.beta == 1000.0
+
+
 def test_projection_files_flip_and_naming(tmp_path):
     nbx, nby = 5, 3
     prj = np.arange(2 * nbx * nby, dtype=np.float32).reshape(2, nbx, nby)
