@@ -10,7 +10,8 @@ resident in HBM before the timed region. Prints ONE JSON line.
 Also reported: per-kernel average durations from HIP events recorded on the
 launch stream, the roofline fraction of the dominant kernel against its
 algorithmic bytes, voxel-ray updates/s, and the CPU oracle timed on this
-host over one subset of the same workload (scaled to s/iter).
+host over one full iteration of the same workload (on the CPUs this job may
+use, and on 1 thread).
 """
 from __future__ import annotations
 
@@ -67,46 +68,79 @@ def profiled_valu(kernel):
         return None
 
 
+def cpu_share():
+    """CPUs this process may use: the affinity mask, capped by a cgroup v2
+    quota (cpu.max) when one is set; plus os.cpu_count() for the record."""
+    n_aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    share = n_aff or os.cpu_count() or 1
+    if quota:
+        share = min(share, quota)
+    return share, {"nproc": os.cpu_count(), "affinity": n_aff, "cgroup_quota": quota,
+                   "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(scan, x_host, y_host, b_host, threads):
-    """Oracle (C++/OpenMP restatement) timed over subset 0 of one iteration."""
+    """Oracle (C++/OpenMP restatement of Tomo_OSTR_gold.cpp:724-887 and the
+    Tomo_OSTR.cu kernels) timed over one full OSTR iteration (all subsets) of
+    the same workload on this host: first the precompute (gamma, yry, d =
+    SBP(gamma yry), :450-538, its d used by the iteration), then one iteration
+    on `threads` threads and one on 1 thread. steady clock, file I/O none."""
     import numpy as np
-    from newrecon_amd.geometry import regroup_index
+    from newrecon_amd.geometry import GAMMA_MODES, regroup_index
     from oracle import oracle as O
 
     g = scan.geom
-    M = scan.recon.n_subsets
-    vs = scan.n_views // M
-    idx = regroup_index(scan.n_views, M)
-    views = idx[:vs]
-    y_sub = np.ascontiguousarray(y_host[views])
-    b_sub = np.ascontiguousarray(b_host[views])
-    d = np.ones(g.vol_shape, np.float32)  # precompute is not part of an iteration
+    cfg = scan.recon
+    M = cfg.n_subsets
+    nv = scan.n_views
+    vs = nv // M
     O.lib()
 
-    def subset_step(nt):
+    t0 = time.perf_counter()
+    gam = O.gamma(g, scan.angles, 0, nv, GAMMA_MODES[cfg.gamma_mode], nthreads=threads)
+    c = O.yry(y_host)
+    d = O.bproj_sbp(g, scan.angles, (gam * c).astype(np.float32), 0, nv, nthreads=threads)
+    t_pre = time.perf_counter() - t0
+    del gam, c
+    idx = regroup_index(nv, M)
+    y_sub = [np.ascontiguousarray(y_host[idx[m * vs:(m + 1) * vs]]) for m in range(M)]
+    b_sub = [np.ascontiguousarray(b_host[idx[m * vs:(m + 1) * vs]]) for m in range(M)]
+
+    def iteration(nt):
+        x = x_host.copy()
         t0 = time.perf_counter()
-        l = O.fproj(g, scan.angles, x_host, 0, vs, index=idx, nthreads=nt)
-        h = O.compute_h(y_sub, b_sub, l)
-        capl = O.bproj_sbp(g, scan.angles, h, 0, vs, index=idx, nthreads=nt)
-        rdd, rd = O.prior(g, x_host, 2, scan.recon.delta, nthreads=nt)
-        O.update_est(x_host, capl, rd, d, rdd, scan.recon.beta, float(M))
+        for m in range(M):
+            a0, a1 = m * vs, (m + 1) * vs
+            l = O.fproj(g, scan.angles, x, a0, a1, index=idx, nthreads=nt)
+            h = O.compute_h(y_sub[m], b_sub[m], l)
+            capl = O.bproj_sbp(g, scan.angles, h, a0, a1, index=idx, nthreads=nt)
+            rdd, rd = O.prior(g, x, 2, cfg.delta, nthreads=nt)
+            x = O.update_est(x, capl, rd, d, rdd, cfg.beta, float(M))
         return time.perf_counter() - t0
 
-    dt = subset_step(threads)
+    dt = iteration(threads)
+    _, share = cpu_share()
     out = {
-        "value": round(dt * M, 3),
+        "value": round(dt, 3),
         "unit": "s/iter",
         "cores": threads,
         "kind": "port",
-        "sample": f"1 of {M} subsets ({vs} views) of one iteration of the same workload, "
-                  f"{dt:.2f} s measured, scaled x{M}",
+        "sample": f"one full iteration ({M} subsets, {nv} views) of the same workload, "
+                  f"d from the oracle's own precompute ({t_pre:.2f} s, not in the value)",
         "cpu_model": cpu_model(),
+        "host": share,
     }
     if threads > 1:
-        d1 = subset_step(1)
-        out["single_thread"] = {"value": round(d1 * M, 3), "unit": "s/iter", "cores": 1,
-                                "sample": f"the same subset on 1 thread, {d1:.2f} s, "
-                                          f"scaled x{M}"}
+        d1 = iteration(1)
+        out["single_thread"] = {"value": round(d1, 3), "unit": "s/iter", "cores": 1,
+                                "sample": "the same iteration on 1 thread"}
     return out
 
 
@@ -323,7 +357,9 @@ def bench_sart(args, dev):
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the CPUs this job may use (affinity and cgroup quota), not the
+        # machine's count: threads beyond the quota only time-slice
+        threads = args.cpu_threads or cpu_share()[0]
         print(f"[bench] CPU oracle baseline (OS-SART subset) on {threads} threads",
               file=sys.stderr, flush=True)
         x_host = rec.x.cpu().numpy()
@@ -440,7 +476,9 @@ def bench_recondbt(args, dev):
     if not args.no_cpu_baseline:
         import numpy as np
         from oracle import recondbt as R
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the CPUs this job may use (affinity and cgroup quota), not the
+        # machine's count: threads beyond the quota only time-slice
+        threads = args.cpu_threads or cpu_share()[0]
         print(f"[bench] CPU oracle (one view of SART_dbt) on {threads} threads",
               file=sys.stderr, flush=True)
         xh = x0.cpu().numpy().astype(np.float64)
@@ -567,7 +605,9 @@ def main(argv=None):
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the CPUs this job may use (affinity and cgroup quota), not the
+        # machine's count: threads beyond the quota only time-slice
+        threads = args.cpu_threads or cpu_share()[0]
         print(f"[bench] CPU oracle baseline on {threads} threads", file=sys.stderr, flush=True)
         x_host = rec.x.float().cpu().numpy()
         cpu = cpu_baseline(scan, x_host, y.cpu().numpy(), b.cpu().numpy(), threads)
