@@ -240,6 +240,20 @@ int ostr_objective(ostr_handle* h, const void* img, int32_t img_dtype, const flo
                    const float* b, const float* r, int32_t potential, double delta,
                    double z_xy_ratio, double* out, void* stream);
 
+/* ostr_objective without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc). Its FP64 partial sums go to
+ * a static per-device scratch array of 2^20 doubles (8 MiB, resolved by
+ * ostr_create, held until the process ends); cases needing more than
+ * (nbx/64 rounded up)(nby/4 rounded up) n_views + (nx/64 rounded up)(ny/4
+ * rounded up) nz + 2 doubles return OSTR_E_RANGE. Same arguments, results and
+ * kernels as ostr_objective, bit for bit. All calls on one device share the
+ * array: calls that overlap on one device (other streams, other threads)
+ * race. */
+int ostr_objective_noalloc(ostr_handle* h, const void* img, int32_t img_dtype, const float* y,
+                           const float* b, const float* r, int32_t potential, double delta,
+                           double z_xy_ratio, double* out, void* stream);
+
 /* out[0] = sum (a_i - b_i)^2, out[1] = sum b_i^2 over n elements, in f64
  * (convergence test and error against a reference image; no reference
  * counterpart). Synchronous; deterministic. */

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -219,6 +220,38 @@ DevGeom make_dev_geom(const ostr_geom& g) {
 
 }  // namespace ostr
 
+// Scratch of the allocation-free objective (ostr_objective_noalloc): the FP64
+// partial sums of the data and penalty terms plus the two results, at most
+// NOALLOC_DOUBLES (8 MiB). Each device holds its own copy for the life of the
+// process. Host code never uses this name as an address: ostr_create resolves
+// the device's copy with hipGetSymbolAddress once per device (g_noalloc_ptr).
+constexpr int64_t NOALLOC_DOUBLES = (int64_t)1 << 20;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch` (resolved on first use per device,
+// with that device current: called from ostr_create).
+int noalloc_resolve(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return fail(nullptr, OSTR_E_UNSUPPORTED, "ostr_create: device %d >= %d", device,
+                    NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return OSTR_OK;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(nullptr, e, "ostr_create: scratch address");
+    if (!p) return fail(nullptr, OSTR_E_HIP, "ostr_create: scratch address is NULL");
+    g_noalloc_ptr[device] = (double*)p;
+    return OSTR_OK;
+}
+
+}  // namespace
+
 // ======================================================================
 OSTR_API int ostr_create(const ostr_geom* geom, const float* angles_rad, const int32_t* index,
                          int32_t n_views, ostr_handle** out) {
@@ -244,6 +277,10 @@ OSTR_API int ostr_create(const ostr_geom* geom, const float* angles_rad, const i
     h->index.resize(n_views);
     for (int i = 0; i < n_views; ++i) h->index[i] = index ? index[i] : i;
     hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess && (rc = noalloc_resolve(h->device)) != OSTR_OK) {
+        ostr_destroy(h);
+        return rc;
+    }
     if (e == hipSuccess) e = hipMalloc(&h->views_dev, sizeof(ostr::ViewTab) * n_views);
     if (e == hipSuccess) e = hipMalloc(&h->views_nat_dev, sizeof(ostr::ViewTab) * n_views);
     if (e == hipSuccess) e = hipMalloc(&h->counter, sizeof(unsigned long long));
@@ -936,25 +973,36 @@ OSTR_API int ostr_count_nnz(ostr_handle* h, int32_t a0, int32_t a1, int64_t* nnz
     return OSTR_OK;
 }
 
-OSTR_API int ostr_objective(ostr_handle* h, const void* img, int32_t img_dtype, const float* y,
-                            const float* b, const float* r, int32_t potential, double delta,
-                            double z_xy_ratio, double* out, void* stream) {
-    CHECK_HANDLE(h);
+// ---- FP64 objective --------------------------------------------------------
+
+namespace {
+
+// Validation shared by both objective entry points.
+int objective_args(ostr_handle* h, int32_t img_dtype, const void* img, const float* y,
+                   const float* b, double* out, int32_t potential, double delta,
+                   const char* fn) {
     int rc;
-    if ((rc = check_dtype(h, img_dtype, "ostr_objective"))) return rc;
-    if (!img || !y || !b || !out) return fail(h, OSTR_E_ARG, "ostr_objective: NULL buffer");
+    if ((rc = check_dtype(h, img_dtype, fn))) return rc;
+    if (!img || !y || !b || !out) return fail(h, OSTR_E_ARG, "%s: NULL buffer", fn);
     if (potential < OSTR_PRIOR_Q2 || potential > OSTR_PRIOR_HYPERBOLA)
-        return fail(h, OSTR_E_ARG, "ostr_objective: unknown potential %d", potential);
+        return fail(h, OSTR_E_ARG, "%s: unknown potential %d", fn, potential);
     if (potential != OSTR_PRIOR_Q2 && delta == 0.0)
-        return fail(h, OSTR_E_ARG, "ostr_objective: delta cannot be zero");
-    if ((rc = use_device(h))) return rc;
-    hipStream_t st = (hipStream_t)stream;
+        return fail(h, OSTR_E_ARG, "%s: delta cannot be zero", fn);
+    return OSTR_OK;
+}
+
+// The objective's launches with `work` holding nd + np + 2 doubles (the data
+// partials, the penalty partials, the two sums); every partial is written by
+// its block, so nothing needs clearing.
+int objective_run(ostr_handle* h, const void* img, int32_t img_dtype, const float* y,
+                  const float* b, const float* r, int32_t potential, double delta,
+                  double z_xy_ratio, double* out, double* work, hipStream_t st,
+                  const char* fn) {
     const int64_t nd = ostr::objective_data_partials(h->dg, h->n_views);
     const int64_t np = ostr::penalty_partials(h->dg);
-    if ((rc = ensure_dwork(h, (size_t)(nd + np + 2), st, "ostr_objective"))) return rc;
-    double* pd = h->dwork;
-    double* pp = h->dwork + nd;
-    double* res = h->dwork + nd + np;
+    double* pd = work;
+    double* pp = work + nd;
+    double* res = work + nd + np;
     int64_t n1 = 0, n2 = 0;
     // all views in natural order: the objective does not depend on the OS regroup
     hipError_t e = ostr::launch_objective_data(h->dg, h->views_nat_dev, h->n_views, img,
@@ -966,10 +1014,58 @@ OSTR_API int ostr_objective(ostr_handle* h, const void* img, int32_t img_dtype, 
     double host[2] = {0.0, 0.0};
     if (e == hipSuccess) e = hipMemcpyAsync(host, res, sizeof host, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(h, e, "ostr_objective");
+    if (e != hipSuccess) return hip_fail(h, e, fn);
     out[0] = host[0];
     out[1] = 0.5 * host[1];
     return OSTR_OK;
+}
+
+}  // namespace
+
+OSTR_API int ostr_objective(ostr_handle* h, const void* img, int32_t img_dtype, const float* y,
+                            const float* b, const float* r, int32_t potential, double delta,
+                            double z_xy_ratio, double* out, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = objective_args(h, img_dtype, img, y, b, out, potential, delta, "ostr_objective")))
+        return rc;
+    if ((rc = use_device(h))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nd = ostr::objective_data_partials(h->dg, h->n_views);
+    const int64_t np = ostr::penalty_partials(h->dg);
+    if ((rc = ensure_dwork(h, (size_t)(nd + np + 2), st, "ostr_objective"))) return rc;
+    return objective_run(h, img, img_dtype, y, b, r, potential, delta, z_xy_ratio, out, h->dwork,
+                         st, "ostr_objective");
+}
+
+OSTR_API int ostr_objective_noalloc(ostr_handle* h, const void* img, int32_t img_dtype,
+                                    const float* y, const float* b, const float* r,
+                                    int32_t potential, double delta, double z_xy_ratio,
+                                    double* out, void* stream) {
+    CHECK_HANDLE(h);
+    int rc;
+    if ((rc = objective_args(h, img_dtype, img, y, b, out, potential, delta,
+                             "ostr_objective_noalloc")))
+        return rc;
+    const int64_t nd = ostr::objective_data_partials(h->dg, h->n_views);
+    const int64_t np = ostr::penalty_partials(h->dg);
+    if (nd + np + 2 > NOALLOC_DOUBLES)
+        return fail(h, OSTR_E_RANGE,
+                    "ostr_objective_noalloc: needs %lld FP64 partials (%lld data + %lld penalty "
+                    "+ 2), the static scratch holds %lld; use ostr_objective",
+                    (long long)(nd + np + 2), (long long)nd, (long long)np,
+                    (long long)NOALLOC_DOUBLES);
+    double* work = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        work = g_noalloc_ptr[h->device];
+    }
+    if (!work)
+        return fail(h, OSTR_E_HIP, "ostr_objective_noalloc: scratch of device %d not resolved",
+                    h->device);
+    if ((rc = use_device(h))) return rc;
+    return objective_run(h, img, img_dtype, y, b, r, potential, delta, z_xy_ratio, out, work,
+                         (hipStream_t)stream, "ostr_objective_noalloc");
 }
 
 OSTR_API int ostr_diff_norms(ostr_handle* h, const void* a, const void* b, int32_t dtype,

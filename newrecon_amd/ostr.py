@@ -84,6 +84,8 @@ def lib():
         "ostr_count_nnz": [vp, i32, i32, ctypes.POINTER(i64), vp],
         "ostr_objective": [vp, vp, i32, vp, vp, vp, i32, ctypes.c_double, ctypes.c_double,
                            ctypes.POINTER(ctypes.c_double), vp],
+        "ostr_objective_noalloc": [vp, vp, i32, vp, vp, vp, i32, ctypes.c_double,
+                                   ctypes.c_double, ctypes.POINTER(ctypes.c_double), vp],
         "ostr_diff_norms": [vp, vp, vp, i32, i64, ctypes.POINTER(ctypes.c_double), vp],
         "ostr_extrapolate": [vp, vp, vp, vp, f32, vp],
         "ostr_restart_dot": [vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double), vp],
