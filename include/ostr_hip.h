@@ -192,6 +192,13 @@ int ostr_bproj_sbp_rows(ostr_handle* h, float* img_accum, const void* prj, int32
 int ostr_precompute_d(ostr_handle* h, float* d, const float* y, const float* r,
                       int32_t gamma_mode, void* stream);
 
+/* ostr_precompute_d with the sum over views in FP64 (the denominator
+ * reduction in double): each view's SBP term in FP32 as ostr_precompute_d
+ * computes it, the terms summed in view order in double, d rounded to f32 once.
+ * Work buffer: nx ny nz (8 + 4) + n_views nbx nby 4 bytes. */
+int ostr_precompute_d_f64(ostr_handle* h, float* d, const float* y, const float* r,
+                          int32_t gamma_mode, void* stream);
+
 /* Forward projection of slots [a0,a1) with the h epilogue: hsub = h(y, b, r,
  * cos_t * A x). y, b, r are the subset's slices, detector-row-major
  * [(a-a0)][nby][nbx] (r may be NULL); hsub (same layout) in prj_dtype; y/b/r
@@ -213,13 +220,15 @@ int ostr_update_fused(ostr_handle* h, const void* x_in, void* x_out, int32_t img
 /* ostr_update_fused with the exact transpose in place of SBP (no reference
  * counterpart): x_out = x_in - (capl_scale (0 + G^T h) + beta RD) /
  * (d + 2 beta RDD), G^T the transpose of ostr_fproj (cos-weighted, as
- * ostr_bproj_matched), the prior of x_in, guard as ostr_update_est. FP32 image
- * and h; h detector-row-major; at most 64 views; x_in and x_out must differ.
- * Equals ostr_bproj_matched + ostr_prior + ostr_update_est bit for bit. */
-int ostr_update_fused_matched(ostr_handle* h, const float* x_in, float* x_out, const float* d,
-                              const float* hsub, int32_t a0, int32_t a1, float beta,
-                              float capl_scale, int32_t potential, float delta, float z_xy_ratio,
-                              int32_t guard, void* stream);
+ * ostr_bproj_matched), the prior of x_in, guard as ostr_update_est. Image (x_in,
+ * x_out) and h each stored as OSTR_F32 / OSTR_BF16 / OSTR_F16 (arithmetic in
+ * FP32, d FP32); h detector-row-major; at most 64 views; x_in and x_out must
+ * differ. With FP32 storage it equals ostr_bproj_matched + ostr_prior +
+ * ostr_update_est bit for bit. */
+int ostr_update_fused_matched(ostr_handle* h, const void* x_in, void* x_out, int32_t img_dtype,
+                              const float* d, const void* hsub, int32_t prj_dtype, int32_t a0,
+                              int32_t a1, float beta, float capl_scale, int32_t potential,
+                              float delta, float z_xy_ratio, int32_t guard, void* stream);
 
 /* ------------------------------------------------------- measurement -- */
 
@@ -363,10 +372,19 @@ int ostr_sart_update_fused(ostr_handle* h, const float* x_in, float* x_out,
  * {t, c, restarts, last test value}, set to {1, 0, 0, 0} by ostr_momentum_init.
  * ostr_momentum_step: dot = <x_k - z, z - z_prev> (FP64, fixed order); dot > 0
  * restarts (t = 1, c = 0, restarts += 1), else t' = (1 + sqrt(1 + 4 t^2)) / 2,
- * c = (t - 1) / t'; then x_next = max(0, z + c (z - z_prev)). Asynchronous. */
+ * c = (t - 1) / t'; then x_next = max(0, z + c (z - z_prev)) in FP32, stored
+ * like the inputs. The four images share one storage type `dtype` (OSTR_F32 /
+ * OSTR_BF16 / OSTR_F16); the dot widens each element exactly to double.
+ * Asynchronous. */
 int ostr_momentum_init(ostr_handle* h, double* state, void* stream);
-int ostr_momentum_step(ostr_handle* h, float* x_next, const float* z, const float* z_prev,
-                       const float* x_k, double* state, void* stream);
+int ostr_momentum_step(ostr_handle* h, void* x_next, const void* z, const void* z_prev,
+                       const void* x_k, int32_t dtype, double* state, void* stream);
+
+/* FP64 accumulation over the n elements (no reference counterpart; the
+ * denominator's sum over views in double): ostr_accum_f64 acc += x,
+ * ostr_f64_to_f32 out = (float) acc (round to nearest). Asynchronous. */
+int ostr_accum_f64(ostr_handle* h, double* acc, const float* x, int64_t n, void* stream);
+int ostr_f64_to_f32(ostr_handle* h, float* out, const double* acc, int64_t n, void* stream);
 
 /* x = max(0, z + c (z - z_prev)) over the volume (f32): the momentum step of
  * accelerated OSTR (Nesterov momentum on the OS / SQS update, Kim, Ramani &

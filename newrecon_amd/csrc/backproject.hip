@@ -495,28 +495,28 @@ __device__ __forceinline__ void update_tile(const DevGeom& g, const SlotTab& S, 
 // launch keeps as many waves as the standalone transpose.
 constexpr int MATCHED_ROWS = 1;
 
-template <int POT>
+template <int POT, typename TI, typename TP>
 __global__ __launch_bounds__(256) void update_fused_matched_kernel(
-    DevGeom g, MatchedSlots S, int n_slots, const float* __restrict__ hsub,
-    const float* __restrict__ x_in, float* __restrict__ x_out, const float* __restrict__ d,
+    DevGeom g, MatchedSlots S, int n_slots, const TP* __restrict__ hsub,
+    const TI* __restrict__ x_in, TI* __restrict__ x_out, const float* __restrict__ d,
     PriorClass W, float inv_delta, float beta, float capl_scale, int guard) {
     constexpr int R = MATCHED_ROWS;
     const TileCoord c = tile_coord<R, CASE_WAVES>(g, xcd_tile(blockIdx.x, gridDim.x));
     if (c.iy0 >= g.ny) return;  // see sbp_kernel
     const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
-    const VolReader<float> P(hsub, (uint32_t)n_slots * g.nbx * g.nby);
-    const VolReader<float> X(x_in, nvox);
-    const VolWriter<float> XO(x_out, nvox);
+    const VolReader<TP> P(hsub, (uint32_t)n_slots * g.nbx * g.nby);
+    const VolReader<TI> X(x_in, nvox);
+    const VolWriter<TI> XO(x_out, nvox);
     const VolReader<float> D(d, nvox);
     float total[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)
         total[r] = (c.ix < g.nx && c.iy0 + r < g.ny)
-                       ? matched_voxel<float>(g, S, n_slots, 1, P, c.ix, c.iy0 + r, c.iz)
+                       ? matched_voxel<TP>(g, S, n_slots, 1, P, c.ix, c.iy0 + r, c.iz)
                        : 0.0f;
     by_prior_mode<R>(g, c, W, [&](auto m) {
-        update_rows<POT, decltype(m)::value, float, R>(g, X, XO, D, c, total, W, inv_delta,
-                                                       beta, capl_scale, guard);
+        update_rows<POT, decltype(m)::value, TI, R>(g, X, XO, D, c, total, W, inv_delta,
+                                                    beta, capl_scale, guard);
     });
 }
 
@@ -761,11 +761,68 @@ hipError_t launch_sart_update_fused(const DevGeom& g, const ViewTab* views, int 
     return hipGetLastError();
 }
 
+namespace {
+
+template <int POT, typename TI, typename TP>
+hipError_t fused_matched_t(const DevGeom& g, const MatchedSlots& S, int n, const void* x_in,
+                           void* x_out, const float* d, const void* hsub, const PriorClass& W,
+                           float inv_delta, float beta, float capl_scale, int guard,
+                           hipStream_t st) {
+    const dim3 grid(volume_tiles(g, MATCHED_ROWS * CASE_WAVES));
+    hipLaunchKernelGGL((update_fused_matched_kernel<POT, TI, TP>), grid, dim3(256), 0, st, g, S,
+                       n, (const TP*)hsub, (const TI*)x_in, (TI*)x_out, d, W, inv_delta, beta,
+                       capl_scale, guard);
+    return hipGetLastError();
+}
+
+template <int POT, typename TI>
+hipError_t fused_matched_h(int prj_dtype, const DevGeom& g, const MatchedSlots& S, int n,
+                           const void* x_in, void* x_out, const float* d, const void* hsub,
+                           const PriorClass& W, float inv_delta, float beta, float capl_scale,
+                           int guard, hipStream_t st) {
+    switch (prj_dtype) {
+        case OSTR_F32:
+            return fused_matched_t<POT, TI, float>(g, S, n, x_in, x_out, d, hsub, W, inv_delta,
+                                                   beta, capl_scale, guard, st);
+        case OSTR_BF16:
+            return fused_matched_t<POT, TI, bf16_t>(g, S, n, x_in, x_out, d, hsub, W, inv_delta,
+                                                    beta, capl_scale, guard, st);
+        case OSTR_F16:
+            return fused_matched_t<POT, TI, _Float16>(g, S, n, x_in, x_out, d, hsub, W,
+                                                      inv_delta, beta, capl_scale, guard, st);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+template <int POT>
+hipError_t fused_matched_x(int img_dtype, int prj_dtype, const DevGeom& g,
+                           const MatchedSlots& S, int n, const void* x_in, void* x_out,
+                           const float* d, const void* hsub, const PriorClass& W,
+                           float inv_delta, float beta, float capl_scale, int guard,
+                           hipStream_t st) {
+    switch (img_dtype) {
+        case OSTR_F32:
+            return fused_matched_h<POT, float>(prj_dtype, g, S, n, x_in, x_out, d, hsub, W,
+                                               inv_delta, beta, capl_scale, guard, st);
+        case OSTR_BF16:
+            return fused_matched_h<POT, bf16_t>(prj_dtype, g, S, n, x_in, x_out, d, hsub, W,
+                                                inv_delta, beta, capl_scale, guard, st);
+        case OSTR_F16:
+            return fused_matched_h<POT, _Float16>(prj_dtype, g, S, n, x_in, x_out, d, hsub, W,
+                                                  inv_delta, beta, capl_scale, guard, st);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
 hipError_t launch_update_fused_matched(const DevGeom& g, const ViewTab* views, int a0, int a1,
-                                       const float* x_in, float* x_out, const float* d,
-                                       const float* hsub, float beta, float capl_scale,
-                                       int potential, float delta, float rho, int guard,
-                                       hipStream_t st) {
+                                       const void* x_in, void* x_out, int img_dtype,
+                                       const float* d, const void* hsub, int prj_dtype,
+                                       float beta, float capl_scale, int potential, float delta,
+                                       float rho, int guard, hipStream_t st) {
     if (a1 - a0 > MAX_SLOTS || a1 < a0) return hipErrorInvalidValue;
     MatchedSlots S;
     for (int a = a0; a < a1; ++a) {
@@ -777,28 +834,23 @@ hipError_t launch_update_fused_matched(const DevGeom& g, const ViewTab* views, i
     prior_weights(W27.w, rho);
     const PriorClass W = prior_params(W27, potential, delta);
     const float inv_delta = potential == OSTR_PRIOR_Q2 ? 0.0f : 1.0f / delta;
-    const dim3 grid(volume_tiles(g, MATCHED_ROWS * CASE_WAVES));
     const int n = a1 - a0;
     switch (potential) {
         case OSTR_PRIOR_Q2:
-            hipLaunchKernelGGL(update_fused_matched_kernel<OSTR_PRIOR_Q2>, grid, dim3(256), 0, st,
-                               g, S, n, hsub, x_in, x_out, d, W, inv_delta, beta, capl_scale,
-                               guard);
-            break;
+            return fused_matched_x<OSTR_PRIOR_Q2>(img_dtype, prj_dtype, g, S, n, x_in, x_out, d,
+                                                  hsub, W, inv_delta, beta, capl_scale, guard,
+                                                  st);
         case OSTR_PRIOR_LANGE:
-            hipLaunchKernelGGL(update_fused_matched_kernel<OSTR_PRIOR_LANGE>, grid, dim3(256), 0,
-                               st, g, S, n, hsub, x_in, x_out, d, W, inv_delta, beta, capl_scale,
-                               guard);
-            break;
+            return fused_matched_x<OSTR_PRIOR_LANGE>(img_dtype, prj_dtype, g, S, n, x_in, x_out,
+                                                     d, hsub, W, inv_delta, beta, capl_scale,
+                                                     guard, st);
         case OSTR_PRIOR_HYPERBOLA:
-            hipLaunchKernelGGL(update_fused_matched_kernel<OSTR_PRIOR_HYPERBOLA>, grid, dim3(256),
-                               0, st, g, S, n, hsub, x_in, x_out, d, W, inv_delta, beta,
-                               capl_scale, guard);
-            break;
+            return fused_matched_x<OSTR_PRIOR_HYPERBOLA>(img_dtype, prj_dtype, g, S, n, x_in,
+                                                         x_out, d, hsub, W, inv_delta, beta,
+                                                         capl_scale, guard, st);
         default:
             return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_sbp_rows(const DevGeom& g, const ViewTab* views, int a0, int a1,

@@ -794,21 +794,43 @@ OSTR_API int ostr_momentum_init(ostr_handle* h, double* state, void* stream) {
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_momentum_init");
 }
 
-OSTR_API int ostr_momentum_step(ostr_handle* h, float* x_next, const float* z,
-                                const float* z_prev, const float* x_k, double* state,
-                                void* stream) {
+OSTR_API int ostr_momentum_step(ostr_handle* h, void* x_next, const void* z,
+                                const void* z_prev, const void* x_k, int32_t dtype,
+                                double* state, void* stream) {
     CHECK_HANDLE(h);
+    int rc;
+    if ((rc = check_dtype(h, dtype, "ostr_momentum_step"))) return rc;
     if (!x_next || !z || !z_prev || !x_k || !state)
         return fail(h, OSTR_E_ARG, "ostr_momentum_step: NULL buffer");
     if (x_next == z || x_next == z_prev || x_next == x_k)
         return fail(h, OSTR_E_ARG, "ostr_momentum_step: x_next aliases an input");
-    int rc;
     if ((rc = use_device(h))) return rc;
     hipStream_t st = (hipStream_t)stream;
     if ((rc = ensure_dwork(h, 1024 + 4, st, "ostr_momentum_step"))) return rc;
     int64_t n = (int64_t)h->geom.nx * h->geom.ny * h->geom.nz;
-    hipError_t e = ostr::launch_momentum_step(x_next, z, z_prev, x_k, n, h->dwork, state, st);
+    hipError_t e =
+        ostr::launch_momentum_step(x_next, z, z_prev, x_k, dtype, n, h->dwork, state, st);
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_momentum_step");
+}
+
+OSTR_API int ostr_accum_f64(ostr_handle* h, double* acc, const float* x, int64_t n,
+                            void* stream) {
+    CHECK_HANDLE(h);
+    if (!acc || !x || n < 0) return fail(h, OSTR_E_ARG, "ostr_accum_f64: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_add_f64(acc, x, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_accum_f64");
+}
+
+OSTR_API int ostr_f64_to_f32(ostr_handle* h, float* out, const double* acc, int64_t n,
+                             void* stream) {
+    CHECK_HANDLE(h);
+    if (!acc || !out || n < 0) return fail(h, OSTR_E_ARG, "ostr_f64_to_f32: bad argument");
+    int rc;
+    if ((rc = use_device(h))) return rc;
+    hipError_t e = ostr::launch_f64_to_f32(out, acc, n, (hipStream_t)stream);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_f64_to_f32");
 }
 
 OSTR_API int ostr_regroup(ostr_handle* h, float* dst, const float* src, int32_t n_subsets,
@@ -876,6 +898,43 @@ OSTR_API int ostr_precompute_d(ostr_handle* h, float* d, const float* y, const f
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_precompute_d");
 }
 
+OSTR_API int ostr_precompute_d_f64(ostr_handle* h, float* d, const float* y, const float* r,
+                                   int32_t gamma_mode, void* stream) {
+    CHECK_HANDLE(h);
+    if (!d || !y) return fail(h, OSTR_E_ARG, "ostr_precompute_d_f64: NULL buffer");
+    if (gamma_mode != OSTR_GAMMA_RAY_LENGTH && gamma_mode != OSTR_GAMMA_ROW_SUM)
+        return fail(h, OSTR_E_ARG, "ostr_precompute_d_f64: unknown gamma mode %d", gamma_mode);
+    int rc;
+    if ((rc = check_sino_span(h, 1, 4, "ostr_precompute_d_f64"))) return rc;
+    if ((rc = use_device(h))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t nbin = (int64_t)h->geom.nbx * h->geom.nby;
+    const int64_t nall = (int64_t)h->n_views * nbin;
+    const int64_t nvox = (int64_t)h->geom.nx * h->geom.ny * h->geom.nz;
+    // work: [acc: nvox doubles][one view's SBP: nvox floats][gamma*yry: nall floats]
+    if ((rc = ensure_work(h, (size_t)(nvox * 8 + (nvox + nall) * 4), st,
+                          "ostr_precompute_d_f64")))
+        return rc;
+    double* acc = (double*)h->work;
+    float* tmp = (float*)(acc + nvox);
+    float* gc = tmp + nvox;
+    // gamma*yry row-major in natural view order, as ostr_precompute_d; then the
+    // views' SBP terms one view at a time, summed in FP64 in view order
+    hipError_t e = ostr::launch_project(h->dg, h->views_nat_dev, 0, h->n_views,
+                                        gamma_mode == OSTR_GAMMA_ROW_SUM ? 5 : 4, 0, nullptr,
+                                        OSTR_F32, gc, OSTR_F32, y, nullptr, r, st);
+    if (e == hipSuccess) e = ostr::launch_fill_f64(acc, 0.0, nvox, st);
+    for (int v = 0; v < h->n_views && e == hipSuccess; ++v) {
+        e = ostr::launch_fill(tmp, 0.0f, nvox, st);
+        if (e == hipSuccess)
+            e = ostr::launch_sbp_rows(h->dg, h->views_nat.data(), v, v + 1, tmp, gc + v * nbin,
+                                      OSTR_F32, st);
+        if (e == hipSuccess) e = ostr::launch_add_f64(acc, tmp, nvox, st);
+    }
+    if (e == hipSuccess) e = ostr::launch_f64_to_f32(d, acc, nvox, st);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_precompute_d_f64");
+}
+
 OSTR_API int ostr_fproj_h(ostr_handle* h, const void* img, int32_t img_dtype, const float* y,
                           const float* b, const float* r, void* hsub, int32_t prj_dtype,
                           int32_t a0, int32_t a1, void* stream) {
@@ -924,14 +983,17 @@ OSTR_API int ostr_update_fused(ostr_handle* h, const void* x_in, void* x_out, in
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_update_fused");
 }
 
-OSTR_API int ostr_update_fused_matched(ostr_handle* h, const float* x_in, float* x_out,
-                                       const float* d, const float* hsub, int32_t a0,
-                                       int32_t a1, float beta, float capl_scale,
-                                       int32_t potential, float delta, float z_xy_ratio,
-                                       int32_t guard, void* stream) {
+OSTR_API int ostr_update_fused_matched(ostr_handle* h, const void* x_in, void* x_out,
+                                       int32_t img_dtype, const float* d, const void* hsub,
+                                       int32_t prj_dtype, int32_t a0, int32_t a1, float beta,
+                                       float capl_scale, int32_t potential, float delta,
+                                       float z_xy_ratio, int32_t guard, void* stream) {
     CHECK_HANDLE(h);
     int rc;
-    if ((rc = check_range(h, a0, a1, "ostr_update_fused_matched"))) return rc;
+    if ((rc = check_range(h, a0, a1, "ostr_update_fused_matched")) ||
+        (rc = check_dtype(h, img_dtype, "ostr_update_fused_matched")) ||
+        (rc = check_dtype(h, prj_dtype, "ostr_update_fused_matched")))
+        return rc;
     if (!x_in || !x_out || !d || !hsub)
         return fail(h, OSTR_E_ARG, "ostr_update_fused_matched: NULL buffer");
     if (x_in == x_out)
@@ -945,12 +1007,13 @@ OSTR_API int ostr_update_fused_matched(ostr_handle* h, const float* x_in, float*
     if (a1 - a0 > ostr::MAX_SLOTS)
         return fail(h, OSTR_E_RANGE, "ostr_update_fused_matched: at most %d views per subset",
                     ostr::MAX_SLOTS);
-    if ((rc = check_sino_span(h, a1 - a0, 4, "ostr_update_fused_matched"))) return rc;
+    if ((rc = check_sino_span(h, a1 - a0, ostr::elem_size(prj_dtype),
+                              "ostr_update_fused_matched")))
+        return rc;
     if ((rc = use_device(h))) return rc;
-    hipError_t e = ostr::launch_update_fused_matched(h->dg, h->views_host.data(), a0, a1, x_in,
-                                                     x_out, d, hsub, beta, capl_scale, potential,
-                                                     delta, z_xy_ratio, guard,
-                                                     (hipStream_t)stream);
+    hipError_t e = ostr::launch_update_fused_matched(
+        h->dg, h->views_host.data(), a0, a1, x_in, x_out, img_dtype, d, hsub, prj_dtype, beta,
+        capl_scale, potential, delta, z_xy_ratio, guard, (hipStream_t)stream);
     return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_update_fused_matched");
 }
 

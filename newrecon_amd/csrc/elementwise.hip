@@ -417,10 +417,12 @@ __global__ __launch_bounds__(256) void diff_norms_kernel(const T* __restrict__ a
 // partials[k] = sum over block k's strided elements of (x - z)(z - zp), FP64
 // (the gradient-restart test of accelerated SQS; fixed grid: deterministic).
 // With state: block 0 also copies the 4 state doubles to snap, which the
-// momentum step's second kernel reads while it rewrites state.
-__global__ __launch_bounds__(256) void restart_dot_kernel(const float* __restrict__ x,
-                                                          const float* __restrict__ z,
-                                                          const float* __restrict__ zp, int64_t n,
+// momentum step's second kernel reads while it rewrites state. T: the
+// images' storage type (each element is widened exactly to double).
+template <typename T>
+__global__ __launch_bounds__(256) void restart_dot_kernel(const T* __restrict__ x,
+                                                          const T* __restrict__ z,
+                                                          const T* __restrict__ zp, int64_t n,
                                                           double* __restrict__ partials,
                                                           const double* __restrict__ state,
                                                           double* __restrict__ snap) {
@@ -428,8 +430,8 @@ __global__ __launch_bounds__(256) void restart_dot_kernel(const float* __restric
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
          i += (int64_t)NORM_BLOCKS * 256) {
-        const double zi = (double)z[i];
-        acc += ((double)x[i] - zi) * (zi - (double)zp[i]);
+        const double zi = (double)ld<T>(z, i);
+        acc += ((double)ld<T>(x, i) - zi) * (zi - (double)ld<T>(zp, i));
     }
     const double s = block_sum_256(acc);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -488,8 +490,9 @@ __global__ __launch_bounds__(256) void momentum_coef_kernel(const double* __rest
 // MOMENTUM_BLOCKS blocks, which bounds the repeated partial sums.
 constexpr int MOMENTUM_BLOCKS = 512;
 
+template <typename T>
 __global__ __launch_bounds__(256) void momentum_extrapolate_kernel(
-    float* __restrict__ x, const float* __restrict__ z, const float* __restrict__ zp,
+    T* __restrict__ x, const T* __restrict__ z, const T* __restrict__ zp,
     const double* __restrict__ p, int64_t n_p, const double* __restrict__ snap,
     double* __restrict__ state, int64_t n) {
     __shared__ float c_blk;
@@ -519,9 +522,54 @@ __global__ __launch_bounds__(256) void momentum_extrapolate_kernel(
     const float c = c_blk;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const float v = fmaf(c, z[i] - zp[i], z[i]);
-        x[i] = v > 0.0f ? v : 0.0f;
+        const float zi = ld<T>(z, i);
+        const float v = fmaf(c, zi - ld<T>(zp, i), zi);
+        st<T>(x, i, v > 0.0f ? v : 0.0f);
     }
+}
+
+// acc += x (FP64 accumulation of FP32 terms: the denominator precompute
+// summed over views in double); out = (float) acc.
+__global__ __launch_bounds__(256) void add_f64_kernel(double* __restrict__ acc,
+                                                      const float* __restrict__ x, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        acc[i] = acc[i] + (double)x[i];
+}
+
+__global__ __launch_bounds__(256) void f64_to_f32_kernel(float* __restrict__ out,
+                                                         const double* __restrict__ acc,
+                                                         int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)acc[i];
+}
+
+__global__ __launch_bounds__(256) void fill_f64_kernel(double* __restrict__ p, double v,
+                                                       int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <typename T>
+hipError_t momentum_step_t(T* x_next, const T* z, const T* z_prev, const T* x_k, int64_t n,
+                           double* partials, double* state, hipStream_t st) {
+    double* snap = partials + NORM_BLOCKS;
+    hipLaunchKernelGGL((restart_dot_kernel<T>), dim3(NORM_BLOCKS), dim3(256), 0, st, x_k, z,
+                       z_prev, n, partials, (const double*)state, snap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (n <= 0) {
+        hipLaunchKernelGGL(momentum_coef_kernel, dim3(1), dim3(256), 0, st, partials,
+                           (int64_t)NORM_BLOCKS, state);
+        return hipGetLastError();
+    }
+    const unsigned blocks = grid_for(n) < MOMENTUM_BLOCKS ? grid_for(n) : MOMENTUM_BLOCKS;
+    hipLaunchKernelGGL((momentum_extrapolate_kernel<T>), dim3(blocks), dim3(256), 0, st, x_next,
+                       z, z_prev, (const double*)partials, (int64_t)NORM_BLOCKS,
+                       (const double*)snap, state, n);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -531,23 +579,38 @@ hipError_t launch_momentum_init(double* state, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_momentum_step(float* x_next, const float* z, const float* z_prev,
-                                const float* x_k, int64_t n, double* partials, double* state,
-                                hipStream_t st) {
-    double* snap = partials + NORM_BLOCKS;
-    hipLaunchKernelGGL(restart_dot_kernel, dim3(NORM_BLOCKS), dim3(256), 0, st, x_k, z, z_prev,
-                       n, partials, (const double*)state, snap);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (n <= 0) {
-        hipLaunchKernelGGL(momentum_coef_kernel, dim3(1), dim3(256), 0, st, partials,
-                           (int64_t)NORM_BLOCKS, state);
-        return hipGetLastError();
+hipError_t launch_momentum_step(void* x_next, const void* z, const void* z_prev,
+                                const void* x_k, int dtype, int64_t n, double* partials,
+                                double* state, hipStream_t st) {
+    switch (dtype) {
+        case OSTR_F32:
+            return momentum_step_t<float>((float*)x_next, (const float*)z, (const float*)z_prev,
+                                          (const float*)x_k, n, partials, state, st);
+        case OSTR_BF16:
+            return momentum_step_t<bf16_t>((bf16_t*)x_next, (const bf16_t*)z,
+                                           (const bf16_t*)z_prev, (const bf16_t*)x_k, n,
+                                           partials, state, st);
+        case OSTR_F16:
+            return momentum_step_t<_Float16>((_Float16*)x_next, (const _Float16*)z,
+                                             (const _Float16*)z_prev, (const _Float16*)x_k, n,
+                                             partials, state, st);
+        default:
+            return hipErrorInvalidValue;
     }
-    const unsigned blocks = grid_for(n) < MOMENTUM_BLOCKS ? grid_for(n) : MOMENTUM_BLOCKS;
-    hipLaunchKernelGGL(momentum_extrapolate_kernel, dim3(blocks), dim3(256), 0, st, x_next, z,
-                       z_prev, (const double*)partials, (int64_t)NORM_BLOCKS,
-                       (const double*)snap, state, n);
+}
+
+hipError_t launch_add_f64(double* acc, const float* x, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(add_f64_kernel, dim3(grid_for(n)), dim3(256), 0, st, acc, x, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_f64_to_f32(float* out, const double* acc, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(grid_for(n)), dim3(256), 0, st, out, acc, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(fill_f64_kernel, dim3(grid_for(n)), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
 
@@ -577,8 +640,8 @@ hipError_t launch_diff_norms(const void* a, const void* b, int dtype, int64_t n,
 
 hipError_t launch_restart_dot(const float* x, const float* z, const float* zp, int64_t n,
                               double* partials, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(restart_dot_kernel, dim3(NORM_BLOCKS), dim3(256), 0, st, x, z, zp, n,
-                       partials, (const double*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL((restart_dot_kernel<float>), dim3(NORM_BLOCKS), dim3(256), 0, st, x, z,
+                       zp, n, partials, (const double*)nullptr, (double*)nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_reduce_partials(partials, NORM_BLOCKS, out, st);

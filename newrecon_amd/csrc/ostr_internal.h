@@ -356,9 +356,14 @@ hipError_t launch_huber_prior(const DevGeom& g, float* prior, const float* img, 
 // round trip): state = {t, c, restarts, last test value}; partials: 1024 + 4
 // doubles (the restart-test partial sums, then a snapshot of state).
 hipError_t launch_momentum_init(double* state, hipStream_t st);
-hipError_t launch_momentum_step(float* x_next, const float* z, const float* z_prev,
-                                const float* x_k, int64_t n, double* partials, double* state,
-                                hipStream_t st);
+// The images in `dtype` storage (F32 / BF16 / F16), arithmetic in FP32 / FP64.
+hipError_t launch_momentum_step(void* x_next, const void* z, const void* z_prev,
+                                const void* x_k, int dtype, int64_t n, double* partials,
+                                double* state, hipStream_t st);
+// FP64 accumulation: acc += x; out = (float) acc; p = v.
+hipError_t launch_add_f64(double* acc, const float* x, int64_t n, hipStream_t st);
+hipError_t launch_f64_to_f32(float* out, const double* acc, int64_t n, hipStream_t st);
+hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t st);
 // FDK-style cosine weighting of views (DBT FBP): pw = D / |S - E| . p.
 hipError_t launch_fbp_weight(const DevGeom& g, const ViewTab* views, int a0, int a1,
                              const float* p, float* pw, hipStream_t st);
@@ -372,13 +377,13 @@ hipError_t launch_fbp_filter(const DevGeom& g, const ViewTab* views, int a0, int
 void fbp_blas_destroy(void* blas);
 hipError_t launch_line_integrals(float* p, const float* y, const float* b, int64_t n,
                                  hipStream_t st);
-// Fused OSTR subset update with the exact transpose (hsub detector-row-major,
-// FP32 image and h).
+// Fused OSTR subset update with the exact transpose (hsub detector-row-major;
+// image and h each in F32 / BF16 / F16 storage, d FP32).
 hipError_t launch_update_fused_matched(const DevGeom& g, const ViewTab* views, int a0, int a1,
-                                       const float* x_in, float* x_out, const float* d,
-                                       const float* hsub, float beta, float capl_scale,
-                                       int potential, float delta, float rho, int guard,
-                                       hipStream_t st);
+                                       const void* x_in, void* x_out, int img_dtype,
+                                       const float* d, const void* hsub, int prj_dtype,
+                                       float beta, float capl_scale, int potential, float delta,
+                                       float rho, int guard, hipStream_t st);
 // ReconDBT SART / ML (SART_dbt.m, ML_dbt.m) elementwise steps.
 hipError_t launch_rdbt_sart_residual(float* y, const float* p, const float* est,
                                      const float* l, float thresh, int64_t n, hipStream_t st);
@@ -465,6 +470,25 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     int full = nblocks - (per - 1) * nxcd;  // XCDs owning `per` tiles
     int t = xcd < full ? xcd * per + slot : full * per + (xcd - full) * (per - 1) + slot;
     return t;
+}
+
+// Work-group to tile mapping for kernels whose tiles cost very different
+// amounts: of a detector's ray tiles only those in the volume's shadow walk
+// (about 15% of the bins at the clinical size, a band across the middle
+// columns), so giving each XCD one contiguous range, as xcd_tile does, leaves
+// the XCDs that own the outer columns idle. Here tiles are grouped in blocks
+// of SPREAD_BLOCK consecutive tiles and the blocks of each run of 8 are dealt
+// one per XCD (block B on XCD B % 8): every XCD gets an even share of the
+// shadow, and the groups of one XCD still work through neighbouring tiles
+// (L2 reuse within a block). Work-group b runs on XCD b % 8; the tiles past
+// the last whole run of 8 blocks keep b's own index.
+constexpr int SPREAD_BLOCK = 16;
+__device__ __forceinline__ int xcd_spread_tile(int b, int nblocks) {
+    constexpr int span = 8 * SPREAD_BLOCK;
+    const int whole = (nblocks / span) * span;
+    if (b >= whole) return b;
+    const int r = b % span;
+    return (b - r) + (r % 8) * SPREAD_BLOCK + r / 8;
 }
 
 // Tile walk of a batched launch (one-tile-per-group and persistent grids

@@ -130,6 +130,9 @@ class OSTRConfig:
     # Nesterov momentum on each full OS pass (accelerated OSTR) with the
     # gradient restart test; off: plain OSTR.
     momentum: bool = False
+    # the denominator d = B(gamma yry) summed over views in FP64 (rounded to
+    # FP32 once) instead of in FP32
+    d_fp64: bool = False
 
     @property
     def potential(self) -> int:

@@ -99,9 +99,12 @@ def lib():
         "ostr_line_integrals": [vp, vp, vp, vp, i64, vp],
         "ostr_fbp_filter": [vp, vp, vp, vp, i32, i32, i32, vp],
         "ostr_momentum_init": [vp, vp, vp],
-        "ostr_update_fused_matched": [vp, vp, vp, vp, vp, i32, i32, f32, f32, i32, f32, f32, i32,
-                                      vp],
-        "ostr_momentum_step": [vp, vp, vp, vp, vp, vp, vp],
+        "ostr_update_fused_matched": [vp, vp, vp, i32, vp, vp, i32, i32, i32, f32, f32, i32, f32,
+                                      f32, i32, vp],
+        "ostr_momentum_step": [vp, vp, vp, vp, vp, i32, vp, vp],
+        "ostr_accum_f64": [vp, vp, vp, i64, vp],
+        "ostr_f64_to_f32": [vp, vp, vp, i64, vp],
+        "ostr_precompute_d_f64": [vp, vp, vp, vp, i32, vp],
         "ostr_rdbt_sart_residual": [vp, vp, vp, vp, vp, f32, i64, vp],
         "ostr_rdbt_sart_update": [vp, vp, vp, vp, f32, i32, i64, vp],
         "ostr_rdbt_ml_terms": [vp, vp, vp, vp, vp, vp, f32, i64, vp],
@@ -290,8 +293,11 @@ class DBTSystem:
     def momentum_step(self, x_next, z, z_prev, x_k, state):
         """Restart test, Nesterov coefficient and extrapolation, all on the
         device (asynchronous): x_next = max(0, z + c (z - z_prev))."""
+        dt = x_next.dtype
+        if any(t.dtype != dt for t in (z, z_prev, x_k)):
+            raise TypeError("momentum_step: the four images must share one dtype")
         self._check(self._L.ostr_momentum_step(self._h, _p(x_next), _p(z), _p(z_prev), _p(x_k),
-                                               _p(state), self._stream()))
+                                               _dt(x_next), _p(state), self._stream()))
         return x_next
 
     def restart_dot(self, x, z, z_prev) -> float:
@@ -301,11 +307,27 @@ class DBTSystem:
                                              ctypes.byref(out), self._stream()))
         return out.value
 
-    def precompute_d(self, y, r=None, gamma_mode="row_sum", d=None):
+    def precompute_d(self, y, r=None, gamma_mode="row_sum", d=None, fp64=False):
+        """d = SBP_all(gamma yry); fp64: the sum over views in double."""
         d = self.volume() if d is None else d
-        self._check(self._L.ostr_precompute_d(self._h, _p(d), _p(y), _p(r),
-                                              GAMMA_MODES[gamma_mode], self._stream()))
+        fn = self._L.ostr_precompute_d_f64 if fp64 else self._L.ostr_precompute_d
+        self._check(fn(self._h, _p(d), _p(y), _p(r), GAMMA_MODES[gamma_mode], self._stream()))
         return d
+
+    def accum_f64(self, acc, x):
+        """acc (float64) += x (float32), elementwise on the device."""
+        if acc.dtype != torch.float64 or acc.numel() != x.numel():
+            raise TypeError("accum_f64: acc must be float64 with x's size")
+        _f32(x, "x")
+        self._check(self._L.ostr_accum_f64(self._h, _p(acc), _p(x), x.numel(), self._stream()))
+        return acc
+
+    def f64_to_f32(self, acc, out=None):
+        out = torch.empty(acc.shape, dtype=torch.float32, device=acc.device) if out is None \
+            else out
+        self._check(self._L.ostr_f64_to_f32(self._h, _p(out), _p(acc), acc.numel(),
+                                            self._stream()))
+        return out
 
     def fproj_h(self, img, y, b, r, a0, a1, hsub=None, prj_dtype=torch.float32):
         hsub = self.sinogram(a1 - a0, prj_dtype) if hsub is None else hsub
@@ -324,13 +346,15 @@ class DBTSystem:
     def update_fused_matched(self, x_in, x_out, d, hsub, a0, a1, beta, capl_scale, potential,
                              delta, z_xy_ratio=None, guard=True):
         """update_fused with the exact transpose: x_out = x_in - (M G^T h + beta RD) /
-        (d + 2 beta RDD); FP32, h detector-row-major."""
+        (d + 2 beta RDD); image and h in F32 / BF16 / F16 storage, d FP32, h
+        detector-row-major."""
         rho = self.geom.dz / self.geom.dx if z_xy_ratio is None else z_xy_ratio
-        for t, what in ((x_in, "x_in"), (x_out, "x_out"), (d, "d"), (hsub, "h")):
-            _f32(t, what)
+        _f32(d, "d")
+        if x_out.dtype != x_in.dtype:
+            raise TypeError("update_fused_matched: x_in and x_out must share one dtype")
         self._check(self._L.ostr_update_fused_matched(
-            self._h, _p(x_in), _p(x_out), _p(d), _p(hsub), a0, a1, beta, capl_scale,
-            POTENTIALS[potential], delta, rho, int(guard), self._stream()))
+            self._h, _p(x_in), _p(x_out), _dt(x_in), _p(d), _p(hsub), _dt(hsub), a0, a1, beta,
+            capl_scale, POTENTIALS[potential], delta, rho, int(guard), self._stream()))
         return x_out
 
     # -- reference names (Tomo_OSTR.cu launch wrappers, gold.cpp host ops) ----
@@ -607,11 +631,10 @@ class OSTR:
             raise ValueError(f"unknown backprojector {self.cfg.backprojector!r}")
         self.matched = self.cfg.backprojector == "matched"
         # fused: two launches per subset (forward + h, then backprojection +
-        # prior + update); the matched fused update is FP32
+        # prior + update), image and h in F32 / BF16 / F16 storage
         self.fused = self.cfg.fused
-        if (not self.fused or self.matched) and (img_dtype != torch.float32
-                                                 or prj_dtype != torch.float32):
-            raise ValueError("BF16/FP16 storage is implemented by the fused SBP kernels only")
+        if not self.fused and (img_dtype != torch.float32 or prj_dtype != torch.float32):
+            raise ValueError("BF16/FP16 storage is implemented by the fused kernels only")
         self.sys = DBTSystem(geom, angles, device=device)
         self.dev = self.sys.device
         self.M, self.vs = M, nv // M
@@ -647,9 +670,8 @@ class OSTR:
                 xi = ml_dbt(geom, angles, self.y_nat, xbp, self.b_nat, 3, 2.0, device=self.dev)
             self.x = xi.to(img_dtype).contiguous()
         else:
-            self.x = dev32(x0).to(img_dtype).contiguous()
-        if self.cfg.momentum and img_dtype != torch.float32:
-            raise ValueError("momentum is implemented for float32 images")
+            # a copy: the iteration works in this buffer, never in the caller's
+            self.x = dev32(x0).to(img_dtype).clone(memory_format=torch.contiguous_format)
         self.iteration = 0
         self.z_prev = None  # momentum state (see _momentum_step)
         self._mstate = None
@@ -679,14 +701,24 @@ class OSTR:
         """gamma, yry, d = SBP(gamma*yry) (natural order), then the OS regroup."""
         s, cfg = self.sys, self.cfg
         nv = self.sys.n_views
-        if self.fused and not self.matched:
-            self.d = s.precompute_d(self.y_nat, self.r_nat, cfg.gamma_mode)
-        elif self.matched:  # d = G^T(yry * G 1)
+        if self.matched:  # d = G^T(yry * G 1)
             gam = s.fproj(s.volume(fill=1.0), 0, nv, cos_weight=True)
             yry = s.compute_yry(torch.empty_like(self.y_nat), self.y_nat, self.r_nat)
             s.compute_gamma_yry(gam, yry, gam)
-            self.d = s.bproj_matched(gam, 0, nv, cos_weight=True)
+            if cfg.d_fp64:  # each view's term in FP32, the sum over views in FP64
+                acc = torch.zeros(self.geom.vol_shape, dtype=torch.float64, device=self.dev)
+                tmp = s.volume()
+                for v in range(nv):
+                    tmp.zero_()
+                    s.accum_f64(acc, s.bproj_matched(gam[v:v + 1], v, v + 1, img=tmp,
+                                                     cos_weight=True))
+                self.d = s.f64_to_f32(acc)
+                del acc, tmp
+            else:
+                self.d = s.bproj_matched(gam, 0, nv, cos_weight=True)
             del gam, yry
+        elif self.fused or cfg.d_fp64:
+            self.d = s.precompute_d(self.y_nat, self.r_nat, cfg.gamma_mode, fp64=cfg.d_fp64)
         else:
             gam = s.gamma(0, nv, cfg.gamma_mode)
             yry = s.compute_yry(torch.empty_like(self.y_nat), self.y_nat, self.r_nat)

@@ -90,8 +90,9 @@ class SQS:
 
 
 def solve(sqs: SQS, x0, iters, tol=1e-10, state=None, log=None, checkpoint=None,
-          every=200):
-    """Accelerated SQS from x0 (or a saved state); returns (z, info)."""
+          every=200, each=None):
+    """Accelerated SQS from x0 (or a saved state); returns (z, info).
+    each(k, z): called with the OS-pass result z after every iteration k."""
     if state is not None:
         x, z, t, k0 = state["x"], state["z"], float(state["t"]), int(state["k"])
     else:
@@ -109,6 +110,8 @@ def solve(sqs: SQS, x0, iters, tol=1e-10, state=None, log=None, checkpoint=None,
             x = np.maximum(zn + ((t - 1.0) / tn) * (zn - z), 0.0)
             t = tn
         z = zn
+        if each is not None:
+            each(k + 1, z)
         if log and (k % every == 0 or res < tol):
             log(f"k={k} step_res={res:.3e} restarts={restarts} t={t:.1f} "
                 f"{time.time() - t0:.0f}s")

@@ -88,3 +88,86 @@ def test_mixed_precision_error_budget(small_case, dev, fp64_reference, oracle_li
     gap = abs(f_gpu - f_ref) / abs(f_ref)
     print(f"{dtype}: objective {f_gpu:.10e} vs FP64 {f_ref:.10e} (rel gap {gap:.2e})")
     assert gap <= bound
+
+
+@pytest.fixture(scope="module")
+def fp64_accelerated(small_case, dev):
+    """The FP64 accelerated SQS solver (oracle/converged.py) from the GPU's
+    backprojected start: its iterates z_k for k = 1..30."""
+    from oracle.converged import SQS, solve
+    s, x, y, b = small_case
+    cfg = OSTRConfig(n_subsets=1, beta=1e3, delta=5e-4, prior="hyperbola",
+                     backprojector="matched", momentum=True)
+    x0 = OSTR(s.geom, s.angles, y, b, cfg=cfg, x0="bp", device=dev).x.double().cpu().numpy()
+    sqs = SQS(s.geom, s.angles, y, b, beta=1e3, delta=5e-4, nthreads=8)
+    snaps = {}
+    solve(sqs, x0, 30, each=lambda k, z: snaps.__setitem__(k, z.copy()))
+    return x0, snaps
+
+
+@pytest.mark.parametrize("img,h,bound", [
+    (torch.float32, torch.float32, 1e-4), (torch.float32, torch.float16, 1e-3),
+    (torch.float32, torch.bfloat16, 5e-3), (torch.float16, torch.float16, 3e-2),
+    (torch.bfloat16, torch.bfloat16, 0.3)])
+def test_mixed_precision_accelerated_matched(small_case, dev, fp64_accelerated, img, h, bound):
+    """The warm-start iteration (one subset, exact transpose, momentum with the
+    device restart test) with image and h stored in 16 bits, rays and updates
+    in FP32, the FP64 reductions unchanged: relative error per iteration
+    against the FP64 solver from the same start."""
+    s, x, y, b = small_case
+    x0, snaps = fp64_accelerated
+    cfg = OSTRConfig(n_subsets=1, beta=1e3, delta=5e-4, prior="hyperbola",
+                     backprojector="matched", momentum=True)
+    rec = OSTR(s.geom, s.angles, y, b, cfg=cfg, x0=x0.astype(np.float32), device=dev,
+               img_dtype=img, prj_dtype=h)
+    errs = []
+
+    def cb(it, xg):
+        ref = snaps[it]
+        xf = xg.float().cpu().numpy().astype(np.float64)
+        errs.append(float(np.linalg.norm(xf - ref) / np.linalg.norm(ref)))
+
+    rec.iterate(30, callback=cb)
+    assert rec.h.dtype == h and rec.estimate.dtype == img
+    print(f"accelerated matched, image {img}, h {h}: rel error vs FP64 solver "
+          + " ".join(f"{e:.1e}" for e in errs[::3]), flush=True)
+    assert max(errs) <= bound, errs
+
+
+def test_fp64_denominator(small_case, dev, oracle_lib):
+    """d summed over views in FP64 (SBP and exact transpose): equal bit for bit
+    to the GPU's own per-view FP32 terms summed in double in view order and
+    rounded once, and within the FP32 bar of the FP64 oracle's d."""
+    s, x, y, b = small_case
+    g = s.geom
+    nv = s.n_views
+    S = DBTSystem(g, s.angles, device=dev)
+    yd = T(y, dev)
+    # the per-view terms: gamma (row sum) * yry, row-major, one view per launch
+    gc = S.gamma(0, nv, "row_sum")
+    S.compute_gamma_yry(gc, S.compute_yry(torch.empty_like(yd), yd), gc)
+    rows = S.to_rows(gc)
+    acc = np.zeros(g.vol_shape, np.float64)
+    for v in range(nv):
+        acc = acc + S.bproj_sbp_rows(rows[v:v + 1], v, v + 1).cpu().numpy().astype(np.float64)
+    d64 = S.precompute_d(yd, None, "row_sum", fp64=True).cpu().numpy()
+    assert np.array_equal(d64, acc.astype(np.float32))
+    gam = oracle_lib.gamma(g, s.angles, 0, nv, 1, dtype=np.float64)
+    c = oracle_lib.yry(y.astype(np.float64), dtype=np.float64)
+    ref = oracle_lib.bproj_sbp(g, s.angles, gam * c, 0, nv, dtype=np.float64)
+    d32 = S.precompute_d(yd, None, "row_sum").cpu().numpy()
+    peak = np.abs(ref).max()
+    e32, e64 = np.abs(d32 - ref).max() / peak, np.abs(d64 - ref).max() / peak
+    print(f"SBP d: FP32 sum {e32:.2e}, FP64 sum {e64:.2e} of the FP64 oracle's peak")
+    assert e64 <= 1e-5
+    # exact transpose, through the OSTR driver
+    refm = oracle_lib.bproj_clip(g, s.angles, c * oracle_lib.fproj(
+        g, s.angles, np.ones(g.vol_shape), 0, nv, dtype=np.float64), 0, nv, nthreads=8)
+    for fp64 in (False, True):
+        cfg = OSTRConfig(n_subsets=3, backprojector="matched", d_fp64=fp64)
+        rec = OSTR(g, s.angles, y, b, cfg=cfg, device=dev)
+        rec.precompute()
+        got = rec.d.cpu().numpy()
+        err = np.abs(got - refm).max() / np.abs(refm).max()
+        print(f"matched d (fp64 sum {fp64}): {err:.2e}")
+        assert err <= 1e-5
