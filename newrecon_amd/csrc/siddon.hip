@@ -9,8 +9,10 @@
 //
 // MI355X mapping: one ray per lane, 64 lanes along detector x (the direction
 // that is contiguous in the volume, so gathers of a wave fall in a few cache
-// lines); a work-group is a 64 x 4 bin tile of one view slot; tiles are dealt
-// to XCDs in contiguous ranges so the frustum a range reads stays in one L2.
+// lines); a work-group is a 64 x 4 bin tile of one view slot; blocks of 16
+// tiles (64 x 64 bins) are dealt round-robin to the XCDs (xcd_spread_tile), so
+// each XCD gets an even share of the rays through the volume and neighbouring
+// tiles of a block share an L2.
 // Per-view trig comes from the handle's table (host FP64, rounded to float).
 // The relative stop test (lmax - at)/lmax > 1e-6 is replaced by `at < thr`
 // with thr computed once per ray as the exact float boundary of that test, so
@@ -186,11 +188,16 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
             for (int k = 0; k < WALK_BATCH; ++k) {
                 const bool act = active;
                 seg[k] = (at - cur) * len;
-                ok[k] = act && inside();
-                off[k] = ok[k] ? px + py + pz : 0u;
-                const bool sx = act && at == X.next;
-                const bool sy = act && at == Y.next;
-                const bool sz = act && at == Z.next;
+                // non-short-circuit & (no lazily evaluated operand, so no
+                // exec-mask branch per step)
+                ok[k] = act & (px < bx) & (py < by) & (pz < bz);
+                // a masked step gathers from past the buffer's end, which
+                // reads 0, so fma(0, seg, sum) leaves sum as it is (seg is
+                // finite and sum never -0): the accumulation needs no select
+                off[k] = ok[k] ? px + py + pz : 0xFFFFFFF0u;
+                const bool sx = act & (at == X.next);
+                const bool sy = act & (at == Y.next);
+                const bool sz = act & (at == Z.next);
                 px += sx ? sxs : 0u;
                 py += sy ? sys : 0u;
                 pz += sz ? szs : 0u;
@@ -200,14 +207,14 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
                 const float nat = fminf(fminf(X.next, Y.next), Z.next);
                 cur = act ? at : cur;
                 at = act ? nat : at;
-                active = act && at < thr;
+                active = act & (at < thr);
             }
             float v[WALK_BATCH];
 #pragma unroll
             for (int k = 0; k < WALK_BATCH; ++k) v[k] = ONES ? 1.0f : img.at_byte(off[k]);
 #pragma unroll
             for (int k = 0; k < WALK_BATCH; ++k) {
-                sum = ok[k] ? fmaf(v[k], seg[k], sum) : sum;
+                sum = (ONES && !ok[k]) ? sum : fmaf(v[k], seg[k], sum);
                 if (NORM) *norm = ok[k] ? fmaf(1.0f, seg[k], *norm) : *norm;
                 if (COUNT && ok[k] && seg[k] > 0.0f) ++nseg;
             }
@@ -319,7 +326,9 @@ __global__ __launch_bounds__(256) void project_kernel(DevGeom g, const ViewTab* 
                                                       const float* __restrict__ b,
                                                       const float* __restrict__ r) {
     int slot, i, j;
-    if (!bin_of_tile<TILE_J>(g, n_slots, xcd_tile(blockIdx.x, gridDim.x), slot, i, j)) return;
+    if (!bin_of_tile<TILE_J>(g, n_slots, xcd_spread_tile(blockIdx.x, gridDim.x), slot, i,
+                             j))
+        return;
     project_bin<KIND, TI, TO>(g, views[a0 + slot], slot, i, j, cos_weight, img, out, y, b, r);
 }
 
@@ -341,7 +350,9 @@ __global__ __launch_bounds__(256) void sart_project_kernel(DevGeom g,
                                                            float* __restrict__ norm,
                                                            const float* __restrict__ p_rows) {
     int slot, i, j;
-    if (!bin_of_tile<TILE_J>(g, n_slots, xcd_tile(blockIdx.x, gridDim.x), slot, i, j)) return;
+    if (!bin_of_tile<TILE_J>(g, n_slots, xcd_spread_tile(blockIdx.x, gridDim.x), slot, i,
+                             j))
+        return;
     const ViewTab v = views[a0 + slot];
     const float ex = bin_centre(g.x_d0, i, g.dbx);
     const float ey = bin_centre(g.y_d0, j, g.dby);
@@ -384,7 +395,7 @@ __global__ __launch_bounds__(256) void project_4r_kernel(DevGeom g,
     __shared__ float sub[4][64];
     const int nbx2 = 2 * g.nbx, nby2 = 2 * g.nby;
     const int tiles_j = (nby2 + 3) / 4;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int t = xcd_spread_tile(blockIdx.x, gridDim.x);
     const int tj = t % tiles_j;
     const int rest = t / tiles_j;
     const int slot = rest % n_slots;
@@ -444,7 +455,7 @@ __global__ __launch_bounds__(256) void count_kernel(DevGeom g, const ViewTab* __
                                                     int a0, int n_slots, int tiles_i,
                                                     int tiles_j,
                                                     unsigned long long* __restrict__ counter) {
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int t = xcd_spread_tile(blockIdx.x, gridDim.x);
     const int tj = t % tiles_j;
     const int rest = t / tiles_j;
     const int slot = rest % n_slots;
