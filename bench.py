@@ -629,9 +629,13 @@ def main(argv=None):
             "warm_s": m["warm"]["seconds"], "warm_iters": m["warm"]["iters"],
             "warm_accepted": m["warm"]["accepted"],
             "speedup": m["speedup"],
-            "rel_start": {"plain": m["plain"]["rel_start"], "warm": m["warm"]["rel_start"]},
+            "warm_mp": {k: m["warm_mp"].get(k) for k in ("storage", "seconds", "iters",
+                                                          "speedup", "floor", "accepted")},
+            "rel_start": {"plain": m["plain"]["rel_start"], "warm": m["warm"]["rel_start"],
+                          "warm_mp": m["warm_mp"]["rel_start"]},
             "rel_after_k": {"plain": m["plain"]["rel_after"], "warm": m["warm"]["rel_after"]},
-            "ostr_m3_floor": m["ostr_m3"],
+            "reference_sbp_m3_floor": m["reference_sbp_m3"],
+            "matched_m3_floor": m["matched_m3"],
             "xstar": m["xstar"],
         }
 

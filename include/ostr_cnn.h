@@ -82,6 +82,15 @@ int ostr_cnn_input(const float* img, int64_t n, float scale, void* x, void* stre
  * estimate), clamped at 0 when clamp != 0. */
 int ostr_cnn_output(const void* out, int32_t ocs, const float* img, int64_t n, float scale,
                     int32_t clamp, float* pred, void* stream);
+/* ostr_cnn_output with the residual low-passed in-plane first (the warm
+ * start's correction without its finest detail):
+ * pred[v] = img[v] + (sum_{j,i} taps[j] taps[i] out[z][y'][x'] * ocs) / scale,
+ * y' = clamp(y + j - radius, 0, ny - 1), x' likewise (the edge repeats), taps
+ * of length 2 radius + 1, radius <= 4; rows summed in order (j outer, i inner)
+ * in FP32; clamped at 0 when clamp != 0. img, pred: [nz][ny][nx] f32. */
+int ostr_cnn_output_smooth(const void* out, int32_t ocs, const float* img, int32_t nz,
+                           int32_t ny, int32_t nx, float scale, const float* taps,
+                           int32_t radius, int32_t clamp, float* pred, void* stream);
 /* Size (doubles) of the ostr_cnn_mse workspace for n voxels. */
 int64_t ostr_cnn_mse_workspace(int64_t n);
 /* Synchronous. *loss = sum (pred - target)^2 (FP64, fixed order); when dout is

@@ -1547,6 +1547,20 @@ OSTR_API int ostr_cnn_output(const void* out, int32_t ocs, const float* img, int
                     "ostr_cnn_output");
 }
 
+OSTR_API int ostr_cnn_output_smooth(const void* out, int32_t ocs, const float* img, int32_t nz,
+                                    int32_t ny, int32_t nx, float scale, const float* taps,
+                                    int32_t radius, int32_t clamp, float* pred, void* stream) {
+    if (!out || !img || !pred || !taps || nz <= 0 || ny <= 0 || nx <= 0 || ocs <= 0 ||
+        scale == 0.0f)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_output_smooth: bad argument");
+    if (radius < 0 || radius > ostr::CNN_SMOOTH_MAX_R)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_output_smooth: radius %d outside 0..%d", radius,
+                        ostr::CNN_SMOOTH_MAX_R);
+    return cnn_done(ostr::launch_cnn_output_smooth(out, ocs, img, nz, ny, nx, 1.0f / scale, taps,
+                                                   radius, clamp, pred, (hipStream_t)stream),
+                    "ostr_cnn_output_smooth");
+}
+
 OSTR_API int64_t ostr_cnn_mse_workspace(int64_t n) { return ostr::cnn_mse_partials(n) + 1; }
 
 OSTR_API int ostr_cnn_mse(const float* pred, const float* target, int64_t n, float gscale,

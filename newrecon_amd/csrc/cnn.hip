@@ -338,6 +338,39 @@ __global__ void output_kernel(const u16* __restrict__ out, int ocs, const float*
     }
 }
 
+// pred[v] = img[v] + (S out)[v] / scale with S the separable in-plane filter
+// sum_{j,i} w[j] w[i] out[z][clamp(y + j - r)][clamp(x + i - r)] (taps w of
+// length 2r + 1, indices clamped to the volume: the edge value repeats); the
+// terms are summed row by row (j outer, i inner) in FP32; clamp at 0 when
+// clamp != 0. The warm start's low-passed residual.
+struct SmoothTaps {
+    float w[2 * CNN_SMOOTH_MAX_R + 1];
+};
+
+__global__ __launch_bounds__(256) void output_smooth_kernel(
+    const u16* __restrict__ out, int ocs, const float* __restrict__ img, int nz, int ny, int nx,
+    float inv_scale, SmoothTaps T, int r, int clamp, float* __restrict__ pred) {
+    const int64_t n = (int64_t)nz * ny * nx;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n;
+         v += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(v % nx);
+        const int y = (int)((v / nx) % ny);
+        const int64_t plane = v - (int64_t)y * nx - x;
+        float acc = 0.0f;
+        for (int j = -r; j <= r; ++j) {
+            const int yy = min(max(y + j, 0), ny - 1);
+            float row = 0.0f;
+            for (int i = -r; i <= r; ++i) {
+                const int xx = min(max(x + i, 0), nx - 1);
+                row = fmaf(T.w[i + r], from_bf16(out[(plane + (int64_t)yy * nx + xx) * ocs]), row);
+            }
+            acc = fmaf(T.w[j + r], row, acc);
+        }
+        const float p = img[v] + acc * inv_scale;
+        pred[v] = (clamp && p < 0.0f) ? 0.0f : p;
+    }
+}
+
 // squared-error partial sums (FP64, one per work-group) and the gradient of
 // mean((pred - target)^2) with respect to out[v][0] (other channels 0).
 __global__ __launch_bounds__(256) void mse_kernel(const float* __restrict__ pred,
@@ -553,6 +586,18 @@ hipError_t launch_cnn_output(const void* out, int ocs, const float* img, int64_t
                              float inv_scale, int clamp, float* pred, hipStream_t st) {
     hipLaunchKernelGGL(output_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const u16*)out, ocs,
                        img, n, inv_scale, clamp, pred);
+    return hipGetLastError();
+}
+
+hipError_t launch_cnn_output_smooth(const void* out, int ocs, const float* img, int nz, int ny,
+                                    int nx, float inv_scale, const float* taps, int r, int clamp,
+                                    float* pred, hipStream_t st) {
+    if (r < 0 || r > CNN_SMOOTH_MAX_R) return hipErrorInvalidValue;
+    SmoothTaps T = {};
+    for (int k = 0; k <= 2 * r; ++k) T.w[k] = taps[k];
+    const int64_t n = (int64_t)nz * ny * nx;
+    hipLaunchKernelGGL(output_smooth_kernel, dim3(grid_for(n)), dim3(256), 0, st,
+                       (const u16*)out, ocs, img, nz, ny, nx, inv_scale, T, r, clamp, pred);
     return hipGetLastError();
 }
 

@@ -332,6 +332,12 @@ hipError_t launch_cnn_up_bwd(const void* dy, int dcs, int dco, void* dx, int xcs
 hipError_t launch_cnn_input(const float* img, int64_t n, float scale, void* x, hipStream_t st);
 hipError_t launch_cnn_output(const void* out, int ocs, const float* img, int64_t n,
                              float inv_scale, int clamp, float* pred, hipStream_t st);
+// Residual output with an in-plane separable filter (taps[2r + 1], r <=
+// CNN_SMOOTH_MAX_R, edge values repeated) applied to out before it is added.
+constexpr int CNN_SMOOTH_MAX_R = 4;
+hipError_t launch_cnn_output_smooth(const void* out, int ocs, const float* img, int nz, int ny,
+                                    int nx, float inv_scale, const float* taps, int r, int clamp,
+                                    float* pred, hipStream_t st);
 hipError_t launch_cnn_mse(const float* pred, const float* target, int64_t n, float gscale,
                           void* dout, int dcs, double* partials, double* result, hipStream_t st);
 hipError_t launch_cnn_pack(const float* w, int cin, int cout, void* wf, void* wd,
@@ -489,6 +495,28 @@ __device__ __forceinline__ int xcd_spread_tile(int b, int nblocks) {
     if (b >= whole) return b;
     const int r = b % span;
     return (b - r) + (r % 8) * SPREAD_BLOCK + r / 8;
+}
+
+// The same spread for a launch that hands each XCD one contiguous range of a
+// case's tile indices (BatchWalk): a permutation of [0, n_tiles) whose eighths
+// cover every eighth block. The whole blocks of SPREAD_BLOCK tiles are put in
+// the order 0, 8, 16, ..., 1, 9, 17, ..., 7, 15, ... (position p -> block);
+// a partial last block keeps its place.
+__device__ __forceinline__ int spread_block_perm(int t, int n_tiles) {
+    const int nb = n_tiles / SPREAD_BLOCK;  // whole blocks
+    const int p = t / SPREAD_BLOCK;
+    if (p >= nb) return t;
+    const int q = nb / 8, rem = nb % 8;    // class k holds q + (k < rem) blocks
+    int k, m;
+    if (p < rem * (q + 1)) {
+        k = p / (q + 1);
+        m = p % (q + 1);
+    } else {
+        const int p2 = p - rem * (q + 1);
+        k = rem + p2 / q;
+        m = p2 % q;
+    }
+    return (k + 8 * m) * SPREAD_BLOCK + t % SPREAD_BLOCK;
 }
 
 // Tile walk of a batched launch (one-tile-per-group and persistent grids

@@ -440,8 +440,11 @@ __global__ __launch_bounds__(64 * TJ) void batch_project_kernel(
         const BatchCase& bc = cases[c];
         const DevGeom g = uniform_copy(bc.g);
         const int n_slots = __builtin_amdgcn_readfirstlane(bc.n_slots);
+        // this XCD's contiguous share of the case's tiles, spread over the
+        // detector (only the shadow's rays walk)
+        const int n_tiles = ((g.nbx + TILE_I - 1) / TILE_I) * ((g.nby + TJ - 1) / TJ) * n_slots;
         int slot, i, j;
-        const bool in = bin_of_tile<TJ>(g, n_slots, t, slot, i, j);
+        const bool in = bin_of_tile<TJ>(g, n_slots, spread_block_perm(t, n_tiles), slot, i, j);
         const ViewTab v = uniform_copy(bc.v[slot]);  // slot is uniform over the group
         if (in)
             project_bin<3, TI, TO>(g, v, slot, i, j, 1, (const TI*)uniform_ptr(bc.x_in),

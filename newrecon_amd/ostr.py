@@ -680,6 +680,20 @@ class OSTR:
         # optional per-kernel timing: name -> list of (start, end) torch.cuda.Events
         self.events = None
 
+    def reset(self, x0: torch.Tensor):
+        """Restart from image x0 (copied, in the image storage type): iteration
+        count, momentum state and the recorded graph are cleared; the
+        precompute is kept."""
+        self.x = x0.to(device=self.dev, dtype=self.img_dtype).clone(
+            memory_format=torch.contiguous_format)
+        if hasattr(self, "x_next"):
+            self.x_next = torch.empty_like(self.x)
+        self.iteration = 0
+        self.z_prev = None
+        self._mstate = None
+        self._xk = None
+        self._graph = None
+
     def _timed(self, name, fn, *args, **kw):
         if self.events is None:
             return fn(*args, **kw)

@@ -7,13 +7,17 @@ tests/golden/xstar_small.npz with the SHA-256 of the counts it was computed
 from. Measured here, on the GPU, from data resident in HBM:
 
 * plain: OSTR with the exact transpose, one subset, Nesterov momentum
-  (accelerated SQS), from the one-step backprojected image;
-* warm:  the same iteration from the network's estimate (accepted only if the
-  FP64 objective is lower there than at the backprojection, else the plain
-  start is used: the fallback);
-* the reference's own iteration, plain OSTR with M = 3 subsets, which ends in
-  an ordered-subsets limit cycle and does not reach the tolerance: its floor
-  is reported.
+  (accelerated SQS), FP32, from the one-step backprojected image;
+* warm: the same iteration from the network's estimate, its residual
+  low-passed in-plane (accepted only if, after one plain pass from each
+  start, the FP64 objective is lower from the network's image than from the
+  backprojection; else the plain start is used: the fallback);
+* warm_mp: the warm path with the image and h stored in 16 bits (rays,
+  updates and momentum in FP32, the objective, restart test and distance in
+  FP64); when it does not reach the tolerance, the error floor it settles at;
+* the reference's own iteration, plain OSTR with the SBP backprojector and
+  M = 3 subsets, and OS-SQS with the exact transpose and M = 3: both end in an
+  ordered-subsets limit cycle; the error after m3_iters is reported.
 
 Each path first runs with a distance check after every iteration (FP64 GPU
 reduction) to find the iteration count; the wall-clock is then taken on a
@@ -57,11 +61,15 @@ def _sync():
     torch.cuda.synchronize()
 
 
+MP_STORAGE = (torch.float32, torch.float16)  # (image, h) of warm_mp
+
+
 def measure(tol: float = 1e-3, max_iters: int = 20000, device=None, weights: str = WEIGHTS,
-            xstar_path: str = XSTAR, ks=(2, 5, 10), m3_iters: int = 3000) -> dict:
+            xstar_path: str = XSTAR, ks=(2, 5, 10), m3_iters: int = 3000,
+            mp_storage=MP_STORAGE) -> dict:
     from .geometry import OSTRConfig, scan_small
     from .ostr import OSTR
-    from .warmstart import WarmStartNet, make_case, network_image, recon_config
+    from .warmstart import WarmStartNet, make_case, recon_config, warm_started
 
     dev = torch.device(device if device is not None else "cuda:0")
     scan = scan_small()
@@ -74,33 +82,35 @@ def measure(tol: float = 1e-3, max_iters: int = 20000, device=None, weights: str
     wmeta = net.load(weights)
     cfg = recon_config(scan)
 
-    def run(start: str, n_check: Optional[int] = None, n_time: Optional[int] = None):
+    def run(start: str, img=torch.float32, h=torch.float32, n_check: Optional[int] = None,
+            n_time: Optional[int] = None):
         """start 'bp' or 'net'; checks per iteration up to n_check (returns
-        iterations to tol and the error after ks), or times n_time iterations."""
+        iterations to tol, the error after ks and the lowest error reached),
+        or times n_time iterations."""
         _sync()
         t0 = time.perf_counter()
-        rec = OSTR(scan.geom, scan.angles, y, b, cfg=cfg, x0="bp", device=dev)
+        rec = OSTR(scan.geom, scan.angles, y, b, cfg=cfg, x0="bp", device=dev, img_dtype=img,
+                   prj_dtype=h)
         info = {}
         if start == "net":
-            bp = rec.x
-            cand = network_image(net, bp)
-            ob, on = rec.objective(bp), rec.objective(cand)
-            info["accepted"] = on < ob
+            ok, ob, on = warm_started(rec, net)
+            info["accepted"] = ok
             info["obj_bp"], info["obj_net"] = ob, on
-            if on < ob:
-                rec.x = cand
-        rec.precompute()
+        if not rec._precomputed:
+            rec.precompute()
         if n_time is not None:
             rec.iterate(n_time, graph=True)
             _sync()
             info["seconds"] = time.perf_counter() - t0
             return info
-        info["rel_start"] = _rel(rec.sys, rec.x, xs)
+        info["rel_start"] = _rel(rec.sys, rec.x.float(), xs)
         info["rel_after"] = {}
-        hit = None
+        hit, best, best_k, r = None, np.inf, 0, np.inf
         for k in range(1, n_check + 1):
             rec.iterate(1)
-            r = _rel(rec.sys, rec.estimate, xs)
+            r = _rel(rec.sys, rec.estimate.float(), xs)
+            if r < best:
+                best, best_k = r, k
             if k in ks:
                 info["rel_after"][k] = r
             if r < tol and hit is None:
@@ -109,34 +119,46 @@ def measure(tol: float = 1e-3, max_iters: int = 20000, device=None, weights: str
                     break
         info["iters"] = hit
         info["rel_end"] = r
+        info["floor"] = {"rel": best, "at_iter": best_k}
         return info
 
     out = {"tol": tol, "xstar": {k: meta[k] for k in ("iterations", "fixed_point_residual",
                                                         "objective") if k in meta},
            "weights": {k: wmeta.get(k) for k in ("pairs", "target_iters", "epochs")}}
-    for name in ("plain", "warm"):
-        start = "bp" if name == "plain" else "net"
-        chk = run(start, n_check=max_iters)
+    paths = {"plain": ("bp", torch.float32, torch.float32),
+             "warm": ("net", torch.float32, torch.float32),
+             "warm_mp": ("net",) + tuple(mp_storage)}
+    for name, (start, img, h) in paths.items():
+        chk = run(start, img, h, n_check=max_iters)
         res = dict(chk)
+        res["storage"] = {"image": str(img).replace("torch.", ""),
+                          "h": str(h).replace("torch.", "")}
         if chk["iters"] is not None:
-            run(start, n_time=chk["iters"])  # warm-up of code paths
-            res["seconds"] = run(start, n_time=chk["iters"])["seconds"]
+            run(start, img, h, n_time=chk["iters"])  # warm-up of code paths
+            res["seconds"] = run(start, img, h, n_time=chk["iters"])["seconds"]
         else:
             res["seconds"] = None
         out[name] = res
-    p, w = out["plain"]["seconds"], out["warm"]["seconds"]
-    out["speedup"] = (p / w) if (p and w) else None
+    p = out["plain"]["seconds"]
+    for name in ("warm", "warm_mp"):
+        w = out[name]["seconds"]
+        out[name]["speedup"] = (p / w) if (p and w) else None
+    out["speedup"] = out["warm"]["speedup"]
 
-    # the reference iteration (M = 3, plain) and its limit-cycle floor
-    cfg3 = OSTRConfig(n_subsets=3, beta=cfg.beta, delta=cfg.delta, prior=cfg.prior,
-                      backprojector="matched")
-    rec = OSTR(scan.geom, scan.angles, y, b, cfg=cfg3, x0="bp", device=dev)
-    _sync()
-    t0 = time.perf_counter()
-    rec.iterate(m3_iters)
-    _sync()
-    out["ostr_m3"] = {"iters": m3_iters, "seconds": time.perf_counter() - t0,
-                      "rel": _rel(rec.sys, rec.x, xs), "reached": False}
+    # ordered subsets (M = 3), no momentum: the reference's own iteration (SBP
+    # backprojector) and OS-SQS with the exact transpose; both end in a limit cycle
+    for name, bproj in (("reference_sbp_m3", "sbp"), ("matched_m3", "matched")):
+        cfg3 = OSTRConfig(n_subsets=3, beta=cfg.beta, delta=cfg.delta, prior=cfg.prior,
+                          backprojector=bproj)
+        rec = OSTR(scan.geom, scan.angles, y, b, cfg=cfg3, x0="bp", device=dev)
+        rec.precompute()
+        _sync()
+        t0 = time.perf_counter()
+        rec.iterate(m3_iters)
+        _sync()
+        rel = _rel(rec.sys, rec.x, xs)
+        out[name] = {"iters": m3_iters, "seconds": time.perf_counter() - t0, "rel": rel,
+                     "reached": bool(rel < tol), "backprojector": bproj}
     return out
 
 

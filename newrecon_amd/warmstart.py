@@ -19,9 +19,12 @@ and weights on MFMA with FP32 accumulation, FP32 master weights, biases and
 Adam moments; PyTorch only holds the device buffers. Sizes must be multiples
 of 4 (two poolings); `pad4` pads a volume.
 
-The warm start is judged by the plain method: `warm_started` accepts the
-network's image only if the FP64 objective there is below the objective at
-the backprojected image, and otherwise the reconstruction starts from the
+As a warm start the network's residual is low-passed in-plane (a Gaussian of
+WARM_SMOOTH_SIGMA voxels, ostr_cnn_output_smooth) before it is added to the
+backprojection. The warm start is judged by the plain method: `warm_started`
+runs one plain OSTR pass from the network's image and one from the
+backprojection and accepts the network's image only if the FP64 objective
+after its pass is the lower; otherwise the reconstruction starts from the
 backprojection (the fallback). Plain OSTR iterations always follow.
 """
 from __future__ import annotations
@@ -45,6 +48,21 @@ LAYERS: List[Tuple[str, int, int, bool]] = [
     ("out", 16, 16, False),
 ]
 SCALE = 20.0  # attenuation (1/mm) to O(1) network units
+# In-plane Gaussian (sigma in voxels) applied to the network's residual when it
+# serves as a warm start: its finest in-plane detail is not what the iteration
+# needs (with it the time to 1e-3 is longer than from the backprojection)
+WARM_SMOOTH_SIGMA = 1.0
+
+
+def gaussian_taps(sigma: float):
+    """Normalised taps exp(-k^2 / (2 sigma^2)), k = -r..r, r = min(4, ceil(3
+    sigma)), as float32; sigma <= 0 gives the identity (r = 0)."""
+    if sigma <= 0:
+        return np.ones(1, np.float32), 0
+    r = int(min(4, np.ceil(3.0 * sigma)))
+    k = np.arange(-r, r + 1, dtype=np.float64)
+    w = np.exp(-0.5 * (k / sigma) ** 2)
+    return (w / w.sum()).astype(np.float32), r
 
 _sig_done = False
 
@@ -71,6 +89,8 @@ def cnn_lib():
                                                 i32, i32, i32, vp]),
             "ostr_cnn_input": (ctypes.c_int, [vp, i64, f32, vp, vp]),
             "ostr_cnn_output": (ctypes.c_int, [vp, i32, vp, i64, f32, i32, vp, vp]),
+            "ostr_cnn_output_smooth": (ctypes.c_int, [vp, i32, vp, i32, i32, i32, f32, vp, i32,
+                                                      i32, vp, vp]),
             "ostr_cnn_mse_workspace": (i64, [i64]),
             "ostr_cnn_mse": (ctypes.c_int, [vp, vp, i64, f32, vp, i32, vp, i64,
                                             ctypes.POINTER(ctypes.c_double), vp]),
@@ -241,8 +261,10 @@ class WarmStartNet:
                                             _ptr(y), ycs, yco, _ptr(mask), mcs, mco, int(relu),
                                             *B["levels"][lvl], self._stream()))
 
-    def forward(self, img: torch.Tensor, clamp: bool = False) -> torch.Tensor:
-        """pred = network(img) for a float32 [nz, ny, nx] volume (multiples of 4)."""
+    def forward(self, img: torch.Tensor, clamp: bool = False,
+                smooth_sigma: float = 0.0) -> torch.Tensor:
+        """pred = network(img) for a float32 [nz, ny, nx] volume (multiples of 4);
+        smooth_sigma > 0: the residual is low-passed in-plane first."""
         img = img.contiguous()
         B = self._buffers(tuple(img.shape))
         L, st = self._L, self._stream()
@@ -266,8 +288,15 @@ class WarmStartNet:
                                    st))
         self._conv("d1", B["cat1"], 48, 0, B["d1"], 16, 0, 0, B)
         self._conv("out", B["d1"], 16, 0, B["out"], 16, 0, 0, B)
-        self._check(L.ostr_cnn_output(_ptr(B["out"]), 16, _ptr(img), n0, SCALE, int(clamp),
-                                      _ptr(B["pred"]), st))
+        if smooth_sigma > 0:
+            taps, r = gaussian_taps(smooth_sigma)
+            nz, ny, nx = img.shape
+            self._check(L.ostr_cnn_output_smooth(_ptr(B["out"]), 16, _ptr(img), nz, ny, nx,
+                                                 SCALE, taps.ctypes.data_as(ctypes.c_void_p), r,
+                                                 int(clamp), _ptr(B["pred"]), st))
+        else:
+            self._check(L.ostr_cnn_output(_ptr(B["out"]), 16, _ptr(img), n0, SCALE, int(clamp),
+                                          _ptr(B["pred"]), st))
         self._img = img
         return B["pred"].view(img.shape)
 
@@ -416,26 +445,34 @@ def evaluate(net: WarmStartNet, bp, target) -> float:
     return loss.value / bp.numel()
 
 
-def network_image(net: WarmStartNet, bp: torch.Tensor) -> torch.Tensor:
-    """The network's estimate from a backprojected image (clamped at 0)."""
+def network_image(net: WarmStartNet, bp: torch.Tensor,
+                  smooth_sigma: float = WARM_SMOOTH_SIGMA) -> torch.Tensor:
+    """The network's estimate from a backprojected image (clamped at 0), its
+    residual low-passed in-plane with a Gaussian of smooth_sigma voxels."""
     shape = bp.shape
     p = pad4(bp)
-    out = net.forward(p, clamp=True)
+    out = net.forward(p, clamp=True, smooth_sigma=smooth_sigma)
     return out[:shape[0], :shape[1], :shape[2]].contiguous().clone()
 
 
-def warm_started(rec, net: WarmStartNet):
-    """Replace rec's initial image (its backprojection) by the network's
-    estimate when that lowers the FP64 objective; returns (accepted, obj_bp,
-    obj_net). Otherwise rec keeps the backprojection: plain OSTR is the
-    fallback."""
-    bp = rec.x
-    cand = network_image(net, bp)
-    obj_bp, obj_net = rec.objective(bp), rec.objective(cand)
-    if obj_net < obj_bp:
-        rec.x = cand
-        return True, obj_bp, obj_net
-    return False, obj_bp, obj_net
+def warm_started(rec, net: WarmStartNet, judge_iters: int = 1):
+    """Start rec (an OSTR from its backprojection, not yet iterated) from the
+    network's estimate if the plain method prefers it: judge_iters plain passes
+    from each start, then the FP64 objectives compared. Returns (accepted,
+    obj_bp, obj_net), the objectives after those passes; rec is left at its
+    chosen start with no iteration done (the passes are not kept)."""
+    if rec.iteration:
+        raise ValueError("warm_started: the reconstruction has already iterated")
+    bp = rec.x.clone()
+    cand = network_image(net, bp.float()).to(bp.dtype)
+    objs = {}
+    for name, x0 in (("bp", bp), ("net", cand)):
+        rec.reset(x0)
+        rec.iterate(judge_iters)
+        objs[name] = rec.objective()
+    accepted = objs["net"] < objs["bp"]
+    rec.reset(cand if accepted else bp)
+    return accepted, objs["bp"], objs["net"]
 
 
 def main(argv=None):
