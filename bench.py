@@ -273,6 +273,94 @@ def sart_bytes(geom, vs, kernel):
             }[kernel]
 
 
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+
+
+def cnn_flops(shape):
+    """Convolution flops of one training step of the warm-start network on a
+    [nz, ny, nx] volume: per layer 2 V cin cout 27 for the forward, the same
+    for the weight gradient and, except the first layer, for the data
+    gradient (cin as the kernels run it, the first layer's 8 padded channels)."""
+    from newrecon_amd.warmstart import LAYERS
+    nz, ny, nx = shape
+    level = {"e1a": 0, "e1b": 0, "d1": 0, "out": 0, "e2a": 1, "e2b": 1, "d2": 1,
+             "e3a": 2, "e3b": 2}
+    fwd = bwd = 0
+    for name, cin, cout, _ in LAYERS:
+        v = (nz >> level[name]) * (ny >> level[name]) * (nx >> level[name])
+        f = 2 * v * cin * cout * 27
+        fwd += f
+        bwd += f + (0 if name == "e1a" else f)
+    return fwd, bwd
+
+
+def bench_cnn(args, dev):
+    """One training step of the warm-start network (forward, loss and
+    backward, Adam) at its stated training size: a seeded 256x256x48 case
+    (0.1x0.1x1 mm, 15 views over +-7.5 deg, I0 = 5e3), input its backprojected
+    image, target its 50-iteration OSTR image. A step is one forward +
+    backward + Adam; roofline: convolution flops / time against the BF16 MFMA
+    peak (the convolutions run as BF16 MFMA, FP32 accumulation)."""
+    import torch
+
+    from newrecon_amd.geometry import OSTRConfig, scan_train5
+    from newrecon_amd.ostr import OSTR
+    from newrecon_amd.warmstart import WarmStartNet, make_case
+
+    scan = scan_train5()
+    print(f"[bench] workload {scan.name}: phantom, counts, 50-iteration target",
+          file=sys.stderr, flush=True)
+    _, y, b = make_case(scan, scan.phantom_seed, dev)
+    rec = OSTR(scan.geom, scan.angles, y, b, cfg=OSTRConfig(n_subsets=5), x0="bp", device=dev)
+    bp = rec.x.clone()
+    target = rec.iterate(50).clone()
+    del rec
+    net = WarmStartNet(dev, seed=0)
+
+    def step():
+        net.forward(bp)
+        loss = net.loss_backward(target)
+        net.adam_step(lr=1e-3)
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    s_step = (time.perf_counter() - t0) / args.steps
+    fwd, bwd = cnn_flops(tuple(bp.shape))
+    # forward alone (HIP events on the launch stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        net.forward(bp)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / 3
+    tf = (fwd + bwd) / s_step / 1e12
+    out = {
+        "metric": "warm-start network training step (forward + backward + Adam)",
+        "value": round(s_step, 6), "unit": "s/step", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(s_step * 1e3, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (MFMA, fp32 accumulation)",
+        "data": "synthetic (seeded breast phantom, its 50-iteration OSTR image as target)",
+        "config": {"workload": scan.name, "volume": [scan.geom.nx, scan.geom.ny, scan.geom.nz],
+                   "channels": [16, 32, 64]},
+        "forward_ms": round(fwd_ms, 4),
+        "conv_gflop_per_step": round((fwd + bwd) / 1e9, 2),
+        "loss_first_last": [losses[0], losses[-1]],
+        "roofline": {"kernel": "all convolutions of a step", "bound": "mfma",
+                     "achieved": round(tf, 2), "peak": BF16_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tf / BF16_MFMA_PEAK_TFLOPS, 4),
+                     "forward_tflops": round(fwd / (fwd_ms * 1e-3) / 1e12, 2)},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_sart(args, dev):
     """OS-SART on the clinical workload (5 subsets of 3 views, Huber prior,
     lambda 1), fused: per subset the SART forward with row sum and
@@ -527,7 +615,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="clinical",
-                    choices=["clinical", "small", "batch", "sart", "recondbt"])
+                    choices=["clinical", "small", "batch", "sart", "recondbt", "cnn"])
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache (JSON) for --config batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -558,6 +646,8 @@ def main(argv=None):
         return bench_sart(args, dev)
     if args.config == "recondbt":
         return bench_recondbt(args, dev)
+    if args.config == "cnn":
+        return bench_cnn(args, dev)
     scan = SCANS[args.config]()
     g, cfg = scan.geom, scan.recon
     cfg.fused = True

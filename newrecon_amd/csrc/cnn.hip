@@ -145,6 +145,59 @@ __global__ void to_cm_kernel(const u16* __restrict__ src, int cs, int co, int C,
     }
 }
 
+// The same copies through an LDS tile, NCOPY at once: work-group b converts
+// the 32-voxel segment (row b / (nxp / 32), columns 32 (b % (nxp / 32)) ..)
+// of every channel. The segment and its two neighbouring voxels are read
+// channels-last, 16 bytes (8 channels of one voxel) per load, into tile
+// [C16][34] (tile column p holds voxel x0 - 1 + p, zero outside [0, nx));
+// copy s (shift dx = s - 1 when NCOPY is 3, 0 when it is 1) is written
+// channel-major 16 bytes (8 voxels of one channel) per store. C and co must
+// be multiples of 8 (cs too), C16 <= 96. Same values as to_cm_kernel.
+constexpr int TO_CM_MAX_C = 96;
+template <int NCOPY>
+__global__ __launch_bounds__(256) void to_cm_tile_kernel(const u16* __restrict__ src, int cs,
+                                                         int co, int C, int C16,
+                                                         u16* __restrict__ dst, int nz, int ny,
+                                                         int nx, int nxp) {
+    __shared__ u16 tile[TO_CM_MAX_C][34];
+    const int xchunks = nxp / 32;
+    const int row = blockIdx.x / xchunks, x0 = (blockIdx.x % xchunks) * 32;
+    const int64_t rows = (int64_t)nz * ny;
+    const int cg8 = C / 8;
+    for (int t = threadIdx.x; t < 34 * cg8; t += 256) {
+        const int p = t / cg8, cg = t % cg8;
+        const int x = x0 - 1 + p;
+        uint4 v = {0u, 0u, 0u, 0u};
+        if ((unsigned)x < (unsigned)nx)
+            v = *reinterpret_cast<const uint4*>(src + ((int64_t)row * nx + x) * cs + co + cg * 8);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            tile[cg * 8 + 2 * k][p] = (u16)(w[k] & 0xFFFFu);
+            tile[cg * 8 + 2 * k + 1][p] = (u16)(w[k] >> 16);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NCOPY * C16 * 4; t += 256) {
+        const int s = t / (C16 * 4), r = t % (C16 * 4);
+        const int c = r / 4, xx = (r % 4) * 8;
+        const int dxs = NCOPY == 3 ? s - 1 : 0;
+        u16 o[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int xo = x0 + xx + k;
+            o[k] = (c < C && xo < nx) ? tile[c][xx + k + dxs + 1] : (u16)0;
+        }
+        uint4 v;
+        v.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
+        v.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
+        v.z = (uint32_t)o[4] | ((uint32_t)o[5] << 16);
+        v.w = (uint32_t)o[6] | ((uint32_t)o[7] << 16);
+        *reinterpret_cast<uint4*>(dst + ((int64_t)s * C16 + c) * rows * nxp +
+                                  (int64_t)row * nxp + x0 + xx) = v;
+    }
+}
+
 // Partial weight gradient of tap `blockIdx.y` over the rows of split
 // blockIdx.x: part[split][co][tap * CIN + ci]. dy_cm: [COUT16][rows][nxp];
 // xs_cm: 3 shifted copies [3][CIN16][rows][nxp] (copy s holds dx = s - 1).
@@ -515,12 +568,24 @@ hipError_t launch_cnn_wgrad(int cin, int cout, const void* x, int xcs, int xco, 
     u16* dy_cm = (u16*)work;
     u16* xs_cm = dy_cm + plane * cout;
     float* part = (float*)(((uintptr_t)(xs_cm + 3 * plane * ci16) + 255) & ~(uintptr_t)255);
-    hipLaunchKernelGGL(to_cm_kernel, dim3(grid_for(plane * cout)), dim3(256), 0, st,
-                       (const u16*)dy, dcs, dco, cout, cout, dy_cm, nz, ny, nx, nxp, 0);
-    for (int s = 0; s < 3; ++s)
-        hipLaunchKernelGGL(to_cm_kernel, dim3(grid_for(plane * ci16)), dim3(256), 0, st,
-                           (const u16*)x, xcs, xco, cin, ci16, xs_cm + (int64_t)s * ci16 * plane,
-                           nz, ny, nx, nxp, s - 1);
+    const int64_t rows = (int64_t)nz * ny;
+    const bool tiled = cout % 8 == 0 && dco % 8 == 0 && dcs % 8 == 0 && cout <= TO_CM_MAX_C &&
+                       cin % 8 == 0 && xco % 8 == 0 && xcs % 8 == 0 && ci16 <= TO_CM_MAX_C &&
+                       rows * (nxp / 32) <= 0x7fffffff;
+    if (tiled) {
+        const dim3 grid((unsigned)(rows * (nxp / 32)));
+        hipLaunchKernelGGL(to_cm_tile_kernel<1>, grid, dim3(256), 0, st, (const u16*)dy, dcs,
+                           dco, cout, cout, dy_cm, nz, ny, nx, nxp);
+        hipLaunchKernelGGL(to_cm_tile_kernel<3>, grid, dim3(256), 0, st, (const u16*)x, xcs, xco,
+                           cin, ci16, xs_cm, nz, ny, nx, nxp);
+    } else {
+        hipLaunchKernelGGL(to_cm_kernel, dim3(grid_for(plane * cout)), dim3(256), 0, st,
+                           (const u16*)dy, dcs, dco, cout, cout, dy_cm, nz, ny, nx, nxp, 0);
+        for (int s = 0; s < 3; ++s)
+            hipLaunchKernelGGL(to_cm_kernel, dim3(grid_for(plane * ci16)), dim3(256), 0, st,
+                               (const u16*)x, xcs, xco, cin, ci16,
+                               xs_cm + (int64_t)s * ci16 * plane, nz, ny, nx, nxp, s - 1);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
 #define CASE(A, B) \

@@ -190,6 +190,17 @@ def scan_tiny(nx=12, ny=10, nz=6, nbx=24, nby=16, n_views=3) -> Scan:
                 (2, 3), 1, 7, 1007, OSTRConfig(n_subsets=n_views, n_iters=3))
 
 
+def scan_train5(seed: int = 100000) -> Scan:
+    """A training case of the learned warm start at its stated size: 256x256x48
+    voxels of 0.1x0.1x1 mm, 15 views over +-7.5 deg, I0 = 5e3, a 320x224
+    detector of 0.14 mm (the volume's shadow with a margin), 10-20
+    fibroglandular blobs and 5 calcifications; phantom seed `seed`, noise seed
+    seed + 1000; its target is a 50-iteration OSTR (5 subsets)."""
+    g = Geometry.half_cone(256, 256, 48, 0.1, 0.1, 1.0, 320, 224, 0.14, 0.14)
+    return Scan(f"dbt-train-256x256x48-15v-{seed}", g, arc_angles(15, 7.5), 5e3, (10, 20), 5,
+                seed, seed + 1000, OSTRConfig(n_subsets=5, n_iters=50))
+
+
 BATCH_THICKNESS = (30, 40, 45, 50, 55, 60, 70, 80)
 
 
