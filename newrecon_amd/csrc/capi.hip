@@ -3,6 +3,7 @@
 // kernel launchers. No process-global geometry (the reference keeps it in
 // __constant__ symbols, Tomo_OSTR.cu:81-115) and no exit(): every failure is
 // an error code plus a message.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -99,12 +100,49 @@ int check_sino_span(ostr_handle* h, int64_t n_slots, int64_t elem_bytes, const c
     return OSTR_OK;
 }
 
+// The bins whose rays may meet the volume: the central projection of the
+// volume's 8 corners from the source onto the detector plane bounds the
+// shadow of every part of the box on the source's side of the plane; its
+// bin range, widened by 2 bins, clamped to the detector. All bins when a
+// corner is not below the source.
+void view_bbox(const ostr_geom& g, ostr::ViewTab& v) {
+    v.i0 = 0;
+    v.i1 = g.nbx - 1;
+    v.j0 = 0;
+    v.j1 = g.nby - 1;
+    const double sx = v.x0, sy = g.source_y, sz = v.z0, ez = -(double)g.detector_r;
+    const double xs[2] = {g.x_p0, (double)g.x_p0 + (double)g.nx * g.dx};
+    const double ys[2] = {g.y_p0, (double)g.y_p0 + (double)g.ny * g.dy};
+    const double zs[2] = {g.z_p0, (double)g.z_p0 + (double)g.nz * g.dz};
+    double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+    for (int c = 0; c < 8; ++c) {
+        const double cx = xs[c & 1], cy = ys[(c >> 1) & 1], cz = zs[c >> 2];
+        if (!(cz < sz) || !(ez < sz)) return;
+        const double t = (ez - sz) / (cz - sz);
+        const double px = sx + t * (cx - sx), py = sy + t * (cy - sy);
+        if (!std::isfinite(px) || !std::isfinite(py)) return;
+        xmin = std::min(xmin, px);
+        xmax = std::max(xmax, px);
+        ymin = std::min(ymin, py);
+        ymax = std::max(ymax, py);
+    }
+    const double qi0 = std::floor((xmin - g.x_d0) / g.dbx - 0.5) - 2.0;
+    const double qi1 = std::ceil((xmax - g.x_d0) / g.dbx - 0.5) + 2.0;
+    const double qj0 = std::floor((ymin - g.y_d0) / g.dby - 0.5) - 2.0;
+    const double qj1 = std::ceil((ymax - g.y_d0) / g.dby - 0.5) + 2.0;
+    v.i0 = (int)std::max(0.0, std::min(qi0, (double)g.nbx));
+    v.i1 = (int)std::max(-1.0, std::min(qi1, (double)g.nbx - 1));
+    v.j0 = (int)std::max(0.0, std::min(qj0, (double)g.nby));
+    v.j1 = (int)std::max(-1.0, std::min(qj1, (double)g.nby - 1));
+}
+
 ostr::ViewTab view_of(const ostr_geom& g, float theta) {
     ostr::ViewTab v;
     v.s = (float)std::sin((double)theta);
     v.c = (float)std::cos((double)theta);
     v.x0 = g.source_r * v.s;
     v.z0 = g.source_r * v.c;
+    view_bbox(g, v);
     return v;
 }
 

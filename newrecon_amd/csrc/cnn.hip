@@ -198,65 +198,105 @@ __global__ __launch_bounds__(256) void to_cm_tile_kernel(const u16* __restrict__
     }
 }
 
-// Partial weight gradient of tap `blockIdx.y` over the rows of split
-// blockIdx.x: part[split][co][tap * CIN + ci]. dy_cm: [COUT16][rows][nxp];
-// xs_cm: 3 shifted copies [3][CIN16][rows][nxp] (copy s holds dx = s - 1).
-template <int CIN, int COUT>
+// Partial weight gradient over the rows of split blockIdx.x: part[split][co]
+// [tap * CIN + ci]. dy_cm: [COUT16][rows][nxp]; xs_cm: 3 shifted copies
+// [3][CIN16][rows][nxp] (copy s holds dx = s - 1). TX = 1: one tap per
+// work-group (blockIdx.y = tap); TX = 3: the three taps dx = -1, 0, 1 of one
+// (dz, dy) (blockIdx.y = (dz + 1) * 3 + dy + 1) share each dY operand. A wave
+// takes the 32-voxel chunks q = wave, wave + 4, ... of its rows in order and
+// loads chunk q + 4's operands before the MFMAs of chunk q (double buffer);
+// chunks whose shifted row falls outside the volume are skipped (their X is
+// zero). The 4 waves' sums are combined in LDS in a fixed order.
+template <int CIN, int COUT, int TX>
 __global__ __launch_bounds__(256) void wgrad_kernel(const u16* __restrict__ dy_cm,
                                                     const u16* __restrict__ xs_cm,
                                                     float* __restrict__ part, int nz, int ny,
                                                     int nxp, int rows_per_split) {
     constexpr int CI16 = (CIN + 15) / 16 * 16, NCI = CI16 / 16, NCO = COUT / 16;
     constexpr int K = 27 * CIN;
-    __shared__ float red[4][NCO * NCI * 4][64];
+    constexpr int NACC = TX * NCO * NCI;
+    __shared__ float red[4][NACC * 4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 15, kg = lane >> 4;
-    const int tap = blockIdx.y, split = blockIdx.x;
-    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int split = blockIdx.x;
+    const int dz = TX == 1 ? (int)blockIdx.y / 9 - 1 : (int)blockIdx.y / 3 - 1;
+    const int dy = TX == 1 ? ((int)blockIdx.y / 3) % 3 - 1 : (int)blockIdx.y % 3 - 1;
+    const int dx1 = TX == 1 ? (int)blockIdx.y % 3 - 1 : 0;  // the single tap's dx
     const int64_t rows = (int64_t)nz * ny, plane = rows * nxp;
-    const u16* xs = xs_cm + (int64_t)(dx + 1) * CI16 * plane;
-    f32x4 acc[NCO][NCI];
+    f32x4 acc[TX][NCO][NCI];
 #pragma unroll
-    for (int a = 0; a < NCO; ++a)
+    for (int t = 0; t < TX; ++t)
 #pragma unroll
-        for (int c = 0; c < NCI; ++c) acc[a][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int a = 0; a < NCO; ++a)
+#pragma unroll
+            for (int c = 0; c < NCI; ++c) acc[t][a][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const int r0 = split * rows_per_split;
     const int r1 = (int)((int64_t)r0 + rows_per_split < rows ? r0 + rows_per_split : rows);
     const int xchunks = nxp / 32;
-    // chunk q = (row, 32-voxel segment); wave w takes q = w, w + 4, ...
-    for (int q = wave; q < (r1 - r0) * xchunks; q += 4) {
+    const int nq = r1 > r0 ? (r1 - r0) * xchunks : 0;
+    struct Ops {
+        bf16x8 a[NCO], b[TX][NCI];
+        bool ok;
+    };
+    auto load = [&](int q, Ops& o) {
         const int row = r0 + q / xchunks, x0 = (q % xchunks) * 32;
         const int iz = row / ny, iy = row % ny;
         const int zz = iz + dz, yy = iy + dy;
-        if ((unsigned)zz >= (unsigned)nz || (unsigned)yy >= (unsigned)ny) continue;
+        o.ok = (unsigned)zz < (unsigned)nz && (unsigned)yy < (unsigned)ny;
+        if (!o.ok) return;
         const int64_t rs = (int64_t)zz * ny + yy;
-        bf16x8 aop[NCO], bop[NCI];
 #pragma unroll
         for (int a = 0; a < NCO; ++a)
-            aop[a] = load8(dy_cm + ((int64_t)(a * 16 + j) * rows + row) * nxp + x0 + kg * 8);
+            o.a[a] = load8(dy_cm + ((int64_t)(a * 16 + j) * rows + row) * nxp + x0 + kg * 8);
 #pragma unroll
-        for (int c = 0; c < NCI; ++c)
-            bop[c] = load8(xs + ((int64_t)(c * 16 + j) * rows + rs) * nxp + x0 + kg * 8);
+        for (int t = 0; t < TX; ++t) {
+            const u16* xs = xs_cm + (int64_t)((TX == 1 ? dx1 : t - 1) + 1) * CI16 * plane;
+#pragma unroll
+            for (int c = 0; c < NCI; ++c)
+                o.b[t][c] = load8(xs + ((int64_t)(c * 16 + j) * rows + rs) * nxp + x0 + kg * 8);
+        }
+    };
+    auto mma = [&](const Ops& o) {
+        if (!o.ok) return;
+#pragma unroll
+        for (int t = 0; t < TX; ++t)
+#pragma unroll
+            for (int a = 0; a < NCO; ++a)
+#pragma unroll
+                for (int c = 0; c < NCI; ++c)
+                    acc[t][a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.a[a], o.b[t][c],
+                                                                          acc[t][a][c], 0, 0, 0);
+    };
+    Ops o0, o1;
+    int q = wave;
+    if (q < nq) load(q, o0);
+    while (q < nq) {
+        if (q + 4 < nq) load(q + 4, o1);
+        mma(o0);
+        q += 4;
+        if (q >= nq) break;
+        if (q + 4 < nq) load(q + 4, o0);
+        mma(o1);
+        q += 4;
+    }
+    // combine the 4 waves in a fixed order
+#pragma unroll
+    for (int t = 0; t < TX; ++t)
 #pragma unroll
         for (int a = 0; a < NCO; ++a)
 #pragma unroll
             for (int c = 0; c < NCI; ++c)
-                acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop[a], bop[c], acc[a][c], 0,
-                                                                    0, 0);
-    }
-    // combine the 4 waves in a fixed order
 #pragma unroll
-    for (int a = 0; a < NCO; ++a)
-#pragma unroll
-        for (int c = 0; c < NCI; ++c)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) red[wave][(a * NCI + c) * 4 + r][lane] = acc[a][c][r];
+                for (int r = 0; r < 4; ++r)
+                    red[wave][((t * NCO + a) * NCI + c) * 4 + r][lane] = acc[t][a][c][r];
     __syncthreads();
     float* out = part + (int64_t)split * COUT * K;
-    for (int e = threadIdx.x; e < NCO * NCI * 4 * 64; e += 256) {
+    for (int e = threadIdx.x; e < NACC * 4 * 64; e += 256) {
         const int slot = e / 64, l = e % 64;
         const float s = ((red[0][slot][l] + red[1][slot][l]) + red[2][slot][l]) + red[3][slot][l];
-        const int a = slot / (NCI * 4), c = (slot / 4) % NCI, r = slot % 4;
+        const int t = slot / (NCO * NCI * 4), a = (slot / (NCI * 4)) % NCO, c = (slot / 4) % NCI;
+        const int r = slot % 4;
+        const int tap = TX == 1 ? (int)blockIdx.y : (int)blockIdx.y * 3 + t;
         const int co = a * 16 + (l >> 4) * 4 + r;  // D row
         const int ci = c * 16 + (l & 15);          // D column
         if (ci < CIN) out[(int64_t)co * K + tap * CIN + ci] = s;
@@ -508,8 +548,12 @@ hipError_t wgrad_t(const u16* dy_cm, const u16* xs_cm, float* part, int nsplit, 
                    int nxp, hipStream_t st) {
     const int rows = nz * ny;
     const int per = (rows + nsplit - 1) / nsplit;
-    hipLaunchKernelGGL((wgrad_kernel<CIN, COUT>), dim3(nsplit, 27), dim3(256), 0, st, dy_cm,
-                       xs_cm, part, nz, ny, nxp, per);
+    // the three x-taps per work-group while their accumulators and double
+    // buffered operands stay small
+    constexpr int NMMA = (COUT / 16) * ((CIN + 15) / 16);
+    constexpr int TX = NMMA <= 4 ? 3 : 1;
+    hipLaunchKernelGGL((wgrad_kernel<CIN, COUT, TX>), dim3(nsplit, 27 / TX), dim3(256), 0, st,
+                       dy_cm, xs_cm, part, nz, ny, nxp, per);
     return hipGetLastError();
 }
 

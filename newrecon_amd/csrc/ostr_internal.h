@@ -20,10 +20,18 @@ struct DevGeom {
     float last_x, last_y, last_z;// fma(n-1, h, p0): lower face of the last voxel
 };
 
-// Per-slot view table entry: source x, source z, cos(theta), sin(theta).
+// Per-slot view table entry: source x, source z, cos(theta), sin(theta), and
+// the detector bins [i0, i1] x [j0, j1] whose rays may meet the volume (the
+// bounding box of its 8 corners projected from the source onto the detector
+// plane, widened by 2 bins; rays to bins outside it miss the volume, so the
+// ray kernels skip their walk there). An empty box (i0 > i1) is allowed.
 struct ViewTab {
     float x0, z0, c, s;
+    int i0, i1, j0, j1;
 };
+__device__ __forceinline__ bool view_may_hit(const ViewTab& v, int i, int j) {
+    return i >= v.i0 && i <= v.i1 && j >= v.j0 && j <= v.j1;
+}
 
 constexpr int MAX_SLOTS = 64;  // views per backprojection pass
 

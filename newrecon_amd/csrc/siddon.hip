@@ -258,18 +258,20 @@ __device__ __forceinline__ void project_bin(const DevGeom& g, const ViewTab& v, 
     const float ex = bin_centre(g.x_d0, i, g.dbx);
     const float ey = bin_centre(g.y_d0, j, g.dby);
     const float ez = -g.det_r;
-    bool hit;
-    float len;
+    bool hit = false;
+    float len = 0.0f;
     int nseg = 0;
-    float s;
+    float s = 0.0f;
     const VolReader<TI> vol(img, (uint32_t)g.nx * g.ny * g.nz);
-    if (KIND == 0 || KIND == 3)
-        s = walk<false, false, TI>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
-    else if (KIND == 2 || KIND == 5)
-        s = walk<true, false, TI>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
-    else {
-        walk<true, false, TI>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
-        s = len;
+    if (view_may_hit(v, i, j)) {  // else the ray misses the volume
+        if (KIND == 0 || KIND == 3)
+            s = walk<false, false, TI>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
+        else if (KIND == 2 || KIND == 5)
+            s = walk<true, false, TI>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
+        else {
+            walk<true, false, TI>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len, nseg);
+            s = len;
+        }
     }
     float val;
     if (!hit)
@@ -358,11 +360,12 @@ __global__ __launch_bounds__(256) void sart_project_kernel(DevGeom g,
     const float ey = bin_centre(g.y_d0, j, g.dby);
     const float ez = -g.det_r;
     const VolReader<float> vol(img, (uint32_t)g.nx * g.ny * g.nz);
-    bool hit;
-    float len, nrm;
+    bool hit = false;
+    float len, nrm = 0.0f, s = 0.0f;
     int nseg = 0;
-    const float s = walk<false, false, float, true>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez,
-                                                    hit, len, nseg, &nrm);
+    if (view_may_hit(v, i, j))
+        s = walk<false, false, float, true>(g, vol, v.x0, g.src_y, v.z0, ex, ey, ez, hit, len,
+                                            nseg, &nrm);
     const float est = hit ? (cos_weight ? s * v.c : s) : 0.0f;
     nrm = hit ? nrm : 0.0f;
     if (DIFF) {
@@ -406,13 +409,15 @@ __global__ __launch_bounds__(256) void project_4r_kernel(DevGeom g,
     if (ii < nbx2 && jj < nby2) {
         const ViewTab v = views[a0 + slot];
         const VolReader<float> vol(img, (uint32_t)g.nx * g.ny * g.nz);
-        bool hit;
-        float len;
+        bool hit = false;
+        float len, s = 0.0f;
         int nseg = 0;
-        const float s = walk<false, false, float>(g, vol, v.x0, g.src_y, v.z0,
-                                                  bin_centre(g.x_d0, ii, 0.5f * g.dbx),
-                                                  bin_centre(g.y_d0, jj, 0.5f * g.dby), -g.det_r,
-                                                  hit, len, nseg);
+        // sub-bin (ii, jj) lies in bin (ii / 2, jj / 2); the box's 2-bin margin covers it
+        if (view_may_hit(v, ii >> 1, jj >> 1))
+            s = walk<false, false, float>(g, vol, v.x0, g.src_y, v.z0,
+                                          bin_centre(g.x_d0, ii, 0.5f * g.dbx),
+                                          bin_centre(g.y_d0, jj, 0.5f * g.dby), -g.det_r, hit,
+                                          len, nseg);
         val = hit ? s : 0.0f;
         if (prj4) prj4[((int64_t)slot * nby2 + jj) * nbx2 + ii] = val;
     }
@@ -471,9 +476,9 @@ __global__ __launch_bounds__(256) void count_kernel(DevGeom g, const ViewTab* __
         bool hit;
         float len;
         const VolReader<float> none(nullptr, 0u);
-        walk<true, true, float>(g, none, v.x0, g.src_y, v.z0,
-                                bin_centre(g.x_d0, i, g.dbx), bin_centre(g.y_d0, j, g.dby),
-                                -g.det_r, hit, len, nseg);
+        if (view_may_hit(v, i, j))
+            walk<true, true, float>(g, none, v.x0, g.src_y, v.z0, bin_centre(g.x_d0, i, g.dbx),
+                                    bin_centre(g.y_d0, j, g.dby), -g.det_r, hit, len, nseg);
     }
     // wave reduction, one atomic per wave
     unsigned long long c = (unsigned long long)nseg;
@@ -662,13 +667,12 @@ __global__ __launch_bounds__(256) void objective_data_kernel(
     if (i < g.nbx && j < g.nby) {
         const ViewTab v = views[slot];
         const VolReader<TI> vol(img, (uint32_t)g.nx * g.ny * g.nz);
-        bool hit;
-        float len;
+        bool hit = false;
+        float len, s = 0.0f;
         int nseg = 0;
-        const float s = walk<false, false, TI>(g, vol, v.x0, g.src_y, v.z0,
-                                               bin_centre(g.x_d0, i, g.dbx),
-                                               bin_centre(g.y_d0, j, g.dby), -g.det_r, hit,
-                                               len, nseg);
+        if (view_may_hit(v, i, j))
+            s = walk<false, false, TI>(g, vol, v.x0, g.src_y, v.z0, bin_centre(g.x_d0, i, g.dbx),
+                                       bin_centre(g.y_d0, j, g.dby), -g.det_r, hit, len, nseg);
         const float l = hit ? s * v.c : 0.0f;
         const int64_t o = ((int64_t)slot * g.nbx + i) * g.nby + j;
         const double yr = (double)b[o] * exp(-(double)l) + (r ? (double)r[o] : 0.0);
