@@ -548,9 +548,11 @@ def bench_recondbt(args, dev):
     k_bt = sum(a.elapsed_time(z) for a, z in ev) / len(ev)
     b_bt = g.n_bin * 4 + 2 * g.n_vox * 4       # one view read; volume read and written
     achieved = b_bt / (k_bt * 1e-3) / 1e9
-    # the FBP initialiser (cosine weights, Toeplitz-GEMM row filter, SBP),
-    # timed with events over the whole call after one warm-up call
-    from newrecon_amd.fbp import fbp_dbt
+    # the FBP initialiser (cosine weight and FFT row filter, SBP), timed with
+    # events over the whole call (handle creation included) after one warm-up
+    # call, and its row filter alone over all views on a resident handle
+    import numpy as np
+    from newrecon_amd.fbp import fbp_dbt, filter_taps
     fbp_dbt(g, scan.angles, p, "hann", device=dev)
     fa, fz = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     fa.record()
@@ -558,7 +560,16 @@ def bench_recondbt(args, dev):
     fz.record()
     torch.cuda.synchronize()
     fbp_ms = fa.elapsed_time(fz)
-    print(f"[bench] FBP (hann) {fbp_ms:.2f} ms", file=sys.stderr, flush=True)
+    taps = torch.from_numpy(filter_taps(g.nbx, g.dbx, "hann").astype(np.float32)).to(dev)
+    q = S.fbp_filter(p, taps, 0, scan.n_views)
+    fa.record()
+    for _ in range(5):
+        S.fbp_filter(p, taps, 0, scan.n_views, q=q)
+    fz.record()
+    torch.cuda.synchronize()
+    filt_ms = fa.elapsed_time(fz) / 5
+    print(f"[bench] FBP (hann) {fbp_ms:.2f} ms, row filter {filt_ms:.3f} ms", file=sys.stderr,
+          flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -598,6 +609,7 @@ def bench_recondbt(args, dev):
                    "sart_stepsize": 0.15, "ml_stepsize": 2.0, "x0": "BP image"},
         "ml_dbt_s_per_iter": round(s_ml, 6),
         "fbp_ms": round(fbp_ms, 3),
+        "fbp_filter_ms": round(filt_ms, 4),
         "kernel_ms": {"matched_transpose_one_view": round(k_bt, 4)},
         "roofline": {"kernel": "matched_transpose_one_view", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -308,10 +308,12 @@ int ostr_prior_sart(ostr_handle* h, float* prior, const float* img, float delta,
  * q[a][i][j] = sum_k taps[k] cw(i', j) p[a][i'][j], i' = i + k - (ntaps-1)/2,
  * cw = D / |S - E| the cosine of the ray (S the slot's source, E the bin centre,
  * D the source height above the detector plane), samples off the detector 0.
- * ntaps odd; taps in device memory. The sum over k is evaluated as a GEMM
- * against the Toeplitz matrix of the taps (FP32, rocBLAS), so its order of
- * addition is the library's; the handle's work buffer grows to the weighted
- * views plus nbx^2 floats. */
+ * ntaps odd; taps in device memory (taps with |i' - i| > nbx - 1 never meet
+ * data). The sum is evaluated as an FP32 FFT convolution in LDS over
+ * L = 2^ceil(log2(2 nbx - 1)) points (exact in exact arithmetic; deterministic),
+ * so nbx is limited by the LDS of one work-group: nbx <= 4096 on MI355X
+ * (160 KiB), OSTR_E_RANGE beyond. The handle's work buffer grows to 12 L bytes
+ * (the taps' spectrum and the twiddles). */
 int ostr_fbp_filter(ostr_handle* h, float* q, const float* p, const float* taps, int32_t ntaps,
                     int32_t a0, int32_t a1, void* stream);
 

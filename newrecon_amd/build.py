@@ -29,9 +29,6 @@ HIPFLAGS = [
 ]
 
 
-# rocBLAS: the FBP row filter's Toeplitz GEMM (csrc/fbp.hip)
-LINK = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocblas"]
-
 
 def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -59,7 +56,7 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(jobs, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, *LINK, "-o", tmp]
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -28,7 +28,7 @@ struct ostr_handle {
     ostr::ViewTab* views_dev = nullptr;
     ostr::ViewTab* views_nat_dev = nullptr;
     unsigned long long* counter = nullptr;
-    void* blas = nullptr;  // rocBLAS handle of the FBP filter, created on first use
+    int lds_limit = 0;        // LDS bytes per work-group on this device
     void* work = nullptr;     // grow-on-demand device work buffer
     size_t work_bytes = 0;
     double* dwork = nullptr;  // FP64 partial sums of the reductions (+2 results)
@@ -319,6 +319,9 @@ OSTR_API int ostr_create(const ostr_geom* geom, const float* angles_rad, const i
         ostr_destroy(h);
         return rc;
     }
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&h->lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                  h->device);
     if (e == hipSuccess) e = hipMalloc(&h->views_dev, sizeof(ostr::ViewTab) * n_views);
     if (e == hipSuccess) e = hipMalloc(&h->views_nat_dev, sizeof(ostr::ViewTab) * n_views);
     if (e == hipSuccess) e = hipMalloc(&h->counter, sizeof(unsigned long long));
@@ -347,7 +350,6 @@ OSTR_API void ostr_destroy(ostr_handle* h) {
     if (h->views_dev) (void)hipFree(h->views_dev);
     if (h->views_nat_dev) (void)hipFree(h->views_nat_dev);
     if (h->counter) (void)hipFree(h->counter);
-    ostr::fbp_blas_destroy(h->blas);
     if (h->work) (void)hipFree(h->work);
     if (h->dwork) (void)hipFree(h->dwork);
     for (void* p : h->retired) (void)hipFree(p);
@@ -529,15 +531,19 @@ OSTR_API int ostr_fbp_filter(ostr_handle* h, float* q, const float* p, const flo
     if (!q || !p || !taps || q == p) return fail(h, OSTR_E_ARG, "ostr_fbp_filter: bad buffers");
     if (ntaps <= 0 || ntaps % 2 == 0)
         return fail(h, OSTR_E_ARG, "ostr_fbp_filter: ntaps must be odd and positive");
+    const int64_t L = (int64_t)1 << ostr::fbp_fft_log2(h->geom.nbx);
+    if ((int64_t)(L + L / 2) * 8 > h->lds_limit)
+        return fail(h, OSTR_E_RANGE,
+                    "ostr_fbp_filter: nbx=%d needs a %lld-point FFT, beyond the %d bytes of LDS "
+                    "per work-group",
+                    h->geom.nbx, (long long)L, h->lds_limit);
     if ((rc = use_device(h))) return rc;
     hipStream_t st = (hipStream_t)stream;
-    const int64_t n = (int64_t)(a1 - a0) * h->geom.nbx * h->geom.nby;
-    const int64_t nt = (int64_t)h->geom.nbx * h->geom.nbx;
-    if ((rc = ensure_work(h, (n + nt) * sizeof(float), st, "ostr_fbp_filter"))) return rc;
-    float* pw = (float*)h->work;
-    hipError_t e = ostr::launch_fbp_filter(h->dg, h->views_dev, a0, a1, taps, ntaps, p, pw,
-                                           pw + n, q, &h->blas, st);
-    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_fbp_filter (weights, GEMM)");
+    if ((rc = ensure_work(h, (size_t)(L + L / 2) * sizeof(float2), st, "ostr_fbp_filter")))
+        return rc;
+    hipError_t e = ostr::launch_fbp_filter(h->dg, h->views_dev, a0, a1, taps, ntaps, p,
+                                           (float2*)h->work, q, h->lds_limit, st);
+    return e == hipSuccess ? OSTR_OK : hip_fail(h, e, "ostr_fbp_filter (spectrum, FFT filter)");
 }
 
 // ---- ReconDBT SART / ML steps ------------------------------------------------

@@ -260,40 +260,6 @@ hipError_t launch_line_integrals(float* p, const float* y, const float* b, int64
     return hipGetLastError();
 }
 
-// FDK-style weighting of the views of a DBT FBP, reference layout
-// [slot][nbx][nby]: each sample times the cosine of its ray to the detector
-// normal, D / |S - E| (D = z0 + detector_r, the source height above the
-// detector plane). The row filter that follows is in fbp.hip.
-__global__ __launch_bounds__(256) void fbp_weight_kernel(DevGeom g,
-                                                         const ViewTab* __restrict__ views,
-                                                         int a0, int n_slots,
-                                                         const float* __restrict__ p,
-                                                         float* __restrict__ pw) {
-    const int64_t nbin = (int64_t)g.nbx * g.nby;
-    const int64_t n = nbin * n_slots;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < n;
-         o += (int64_t)gridDim.x * blockDim.x) {
-        const int slot = (int)(o / nbin);
-        const int rem = (int)(o % nbin);
-        const int i = rem / g.nby, j = rem % g.nby;
-        const ViewTab v = views[a0 + slot];
-        const float D = v.z0 + g.det_r;
-        const float ex = (float)((double)g.x_d0 + ((double)i + 0.5) * (double)g.dbx);
-        const float ey = (float)((double)g.y_d0 + ((double)j + 0.5) * (double)g.dby);
-        const float dx = ex - v.x0, dy = ey - g.src_y;
-        pw[o] = (D / sqrtf(fmaf(dx, dx, fmaf(dy, dy, D * D)))) * p[o];
-    }
-}
-
-hipError_t launch_fbp_weight(const DevGeom& g, const ViewTab* views, int a0, int a1,
-                             const float* p, float* pw, hipStream_t st) {
-    const int64_t n = (int64_t)g.nbx * g.nby * (a1 - a0);
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fbp_weight_kernel, dim3(grid_for(n)), dim3(256), 0, st, g, views, a0,
-                       a1 - a0, p, pw);
-    return hipGetLastError();
-}
-
 hipError_t launch_rdbt_sart_residual(float* y, const float* p, const float* est,
                                      const float* l, float thresh, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;

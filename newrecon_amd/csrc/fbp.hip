@@ -1,68 +1,257 @@
-// fbp.hip - the row filter of the DBT FBP as a GEMM on the matrix cores.
+// fbp.hip - the row filter of the DBT FBP: cosine weight and FFT convolution
+// in LDS.
 //
 // The filter is a convolution along detector x (the tube-motion direction)
 // with odd-length centred taps, samples off the detector 0:
-//   q[a][i][j] = sum_k taps[k] pw[a][i + k - c][j],  c = (ntaps - 1) / 2,
-// that is q[a] = T pw[a] with the nbx x nbx Toeplitz matrix
-// T[i][i'] = taps[i' - i + c] (0 outside the taps). With the views in the
-// reference layout [slot][nbx][nby] (j fastest), pw[a] read column-major is
-// the nby x nbx matrix pw[a]^T, so one strided-batched SGEMM forms
-// q[a]^T = pw[a]^T T^T for every view: 2 nby nbx^2 flops per view in FP32 on
-// MFMA (rocBLAS), instead of a direct sum whose per-bin loads made it
-// gather-bound.
-#include <rocblas/rocblas.h>
-
+//   q[a][i][j] = sum_i' taps[i' - i + c] pw[a][i'][j],  c = (ntaps - 1) / 2,
+// pw the cosine-weighted views. Only taps with |i' - i| <= nbx - 1 meet data,
+// so with L the power of two >= 2 nbx - 1 the sum is exactly the circular
+// convolution of the zero-padded column with
+//   H[m mod L] = taps[c - m],  |m| <= nbx - 1 (0 elsewhere),
+// q = IFFT(FFT(pw column) FFT(H)). Two real columns ride in one complex
+// sequence (column j real, j + 1 imaginary): H is real, so the convolution
+// keeps them apart. Per column pair a forward and an inverse radix-2 FFT
+// (L/2 log2 L butterflies each) replace the 2 nbx^2 multiply-adds of the
+// direct sum.
+//
+// fbp_spectrum_kernel (one group) forms FFT(H) / L and the twiddle table
+// exp(-2 pi i k / L), k < L/2, rounded from double; fbp_filter_kernel gives
+// each group SEQ column pairs of one view: it loads them weighted and in
+// bit-reversed order into LDS, runs the decimation-in-time forward transform
+// (natural order out), multiplies by the spectrum, runs the
+// decimation-in-frequency inverse with conjugate twiddles (bit-reversed order
+// out) and stores rows i < nbx from their bit-reversed positions, so no
+// reordering pass is needed.
 #include "ostr_internal.h"
 
 namespace ostr {
 
 namespace {
 
-// T^T column-major = T row-major: element o = i * nbx + i' holds T[i][i'].
-__global__ __launch_bounds__(256) void toeplitz_kernel(const float* __restrict__ taps, int ntaps,
-                                                       int nbx, float* __restrict__ T) {
-    const int64_t n = (int64_t)nbx * nbx;
-    const int c = (ntaps - 1) / 2;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < n;
-         o += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(o / nbx), ip = (int)(o % nbx);
-        const int k = ip - i + c;
-        T[o] = (k >= 0 && k < ntaps) ? taps[k] : 0.0f;
+constexpr int FFT_THREADS = 256;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+    return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+    return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ unsigned bitrev(unsigned i, int logl) {
+    return __builtin_bitreverse32(i) >> (32 - logl);
+}
+
+// Forward transform, bit-reversed order in, natural order out, on the seq
+// sequences of L = 2^logl elements at buf + s L. Ends with a barrier.
+__device__ void fft_dit(float2* buf, int seq, int logl, const float2* tw) {
+    const int L = 1 << logl;
+    for (int lg = 0; lg < logl; ++lg) {
+        const int m = 1 << lg;
+        __syncthreads();
+        for (int b = threadIdx.x; b < L / 2; b += FFT_THREADS) {
+            const int k = b & (m - 1);
+            const int i0 = ((b >> lg) << (lg + 1)) | k;
+            const float2 w = tw[k << (logl - 1 - lg)];
+            for (int s = 0; s < seq; ++s) {
+                float2* a = buf + s * L;
+                const float2 u = a[i0], o = cmul(a[i0 + m], w);
+                a[i0] = cadd(u, o);
+                a[i0 + m] = csub(u, o);
+            }
+        }
     }
+    __syncthreads();
+}
+
+// Inverse transform without the 1/L, natural order in, bit-reversed order out.
+__device__ void fft_dif_inverse(float2* buf, int seq, int logl, const float2* tw) {
+    const int L = 1 << logl;
+    for (int lg = logl - 1; lg >= 0; --lg) {
+        const int m = 1 << lg;
+        __syncthreads();
+        for (int b = threadIdx.x; b < L / 2; b += FFT_THREADS) {
+            const int k = b & (m - 1);
+            const int i0 = ((b >> lg) << (lg + 1)) | k;
+            const float2 t = tw[k << (logl - 1 - lg)];
+            const float2 w = make_float2(t.x, -t.y);
+            for (int s = 0; s < seq; ++s) {
+                float2* a = buf + s * L;
+                const float2 u = a[i0], v = a[i0 + m];
+                a[i0] = cadd(u, v);
+                a[i0 + m] = cmul(csub(u, v), w);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// One group: the twiddles tw (global, L/2) and hs = FFT(H) / L (global, L).
+// LDS: L + L/2 float2.
+__global__ __launch_bounds__(FFT_THREADS) void fbp_spectrum_kernel(const float* __restrict__ taps,
+                                                                   int ntaps, int nbx, int logl,
+                                                                   float2* __restrict__ tw_out,
+                                                                   float2* __restrict__ hs) {
+    extern __shared__ float2 lds[];
+    const int L = 1 << logl, c = (ntaps - 1) / 2;
+    float2* tw = lds;
+    float2* a = lds + L / 2;
+    for (int k = threadIdx.x; k < L / 2; k += FFT_THREADS) {
+        double sn, cs;
+        sincospi(-2.0 * (double)k / (double)L, &sn, &cs);
+        const float2 w = make_float2((float)cs, (float)sn);
+        tw[k] = w;
+        tw_out[k] = w;
+    }
+    for (int p = threadIdx.x; p < L; p += FFT_THREADS) {
+        const int m = p < L / 2 ? p : p - L;
+        const int t = c - m;
+        const bool on = m >= -(nbx - 1) && m <= nbx - 1 && t >= 0 && t < ntaps;
+        a[bitrev((unsigned)p, logl)] = make_float2(on ? taps[t] : 0.0f, 0.0f);
+    }
+    fft_dit(a, 1, logl, tw);
+    const float inv = 1.0f / (float)L;  // a power of two: exact
+    for (int p = threadIdx.x; p < L; p += FFT_THREADS)
+        hs[p] = make_float2(a[p].x * inv, a[p].y * inv);
+}
+
+// Group b of an n-group launch runs on XCD b % 8; hand each XCD one contiguous
+// range of task indices so that neighbouring column chunks, which share cache
+// lines of every detector row, meet in the same L2.
+__device__ __forceinline__ int xcd_contiguous(int b, int n) {
+    const int x = b & 7, q = n >> 3, r = n & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// Group = SEQ column pairs (2 SEQ columns from j0) of one slot. LDS: L/2 + SEQ L
+// float2. p and q are [slot][nbx][nby] from slot a0; vec: nby % 4 == 0 and p,
+// q 16-byte aligned (SEQ = 2: float4 row loads and stores).
+template <int SEQ>
+__global__ __launch_bounds__(FFT_THREADS) void fbp_filter_kernel(
+    DevGeom g, const ViewTab* __restrict__ views, int a0, int n_slots, int n_chunks, int logl,
+    const float2* __restrict__ tw_g, const float2* __restrict__ hs, const float* __restrict__ p,
+    float* __restrict__ q, int vec) {
+    extern __shared__ float2 lds[];
+    const int L = 1 << logl;
+    float2* tw = lds;
+    float2* buf = lds + L / 2;
+    const int task = xcd_contiguous((int)blockIdx.x, n_chunks * n_slots);
+    const int slot = task / n_chunks;
+    const int j0 = (task % n_chunks) * 2 * SEQ;
+    const int nbx = g.nbx, nby = g.nby;
+    const ViewTab v = views[a0 + slot];
+    const float D = v.z0 + g.det_r;
+    float dy[2 * SEQ];
+#pragma unroll
+    for (int c = 0; c < 2 * SEQ; ++c) {
+        const float ey = (float)((double)g.y_d0 + ((double)(j0 + c) + 0.5) * (double)g.dby);
+        dy[c] = ey - g.src_y;
+    }
+    const int64_t base = (int64_t)slot * nbx * nby + j0;
+
+    for (int k = threadIdx.x; k < L / 2; k += FFT_THREADS) tw[k] = tw_g[k];
+    for (int i = threadIdx.x; i < L; i += FFT_THREADS) {
+        float val[2 * SEQ];
+#pragma unroll
+        for (int c = 0; c < 2 * SEQ; ++c) val[c] = 0.0f;
+        if (i < nbx) {
+            const float ex = (float)((double)g.x_d0 + ((double)i + 0.5) * (double)g.dbx);
+            const float dx = ex - v.x0;
+            const float* row = p + base + (int64_t)i * nby;
+            if (SEQ == 2 && vec) {
+                const float4 r4 = *(const float4*)row;
+                val[0] = r4.x;
+                val[1] = r4.y;
+                val[2] = r4.z;
+                val[3] = r4.w;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 2 * SEQ; ++c) val[c] = j0 + c < nby ? row[c] : 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < 2 * SEQ; ++c)
+                val[c] *= D / sqrtf(fmaf(dx, dx, fmaf(dy[c], dy[c], D * D)));
+        }
+        const unsigned r = bitrev((unsigned)i, logl);
+#pragma unroll
+        for (int s = 0; s < SEQ; ++s) buf[s * L + r] = make_float2(val[2 * s], val[2 * s + 1]);
+    }
+    fft_dit(buf, SEQ, logl, tw);
+    for (int f = threadIdx.x; f < L; f += FFT_THREADS) {
+        const float2 h = hs[f];
+#pragma unroll
+        for (int s = 0; s < SEQ; ++s) buf[s * L + f] = cmul(buf[s * L + f], h);
+    }
+    fft_dif_inverse(buf, SEQ, logl, tw);
+    for (int i = threadIdx.x; i < nbx; i += FFT_THREADS) {
+        const unsigned r = bitrev((unsigned)i, logl);
+        float val[2 * SEQ];
+#pragma unroll
+        for (int s = 0; s < SEQ; ++s) {
+            const float2 y = buf[s * L + r];
+            val[2 * s] = y.x;
+            val[2 * s + 1] = y.y;
+        }
+        float* row = q + base + (int64_t)i * nby;
+        if (SEQ == 2 && vec) {
+            *(float4*)row = make_float4(val[0], val[1], val[2], val[3]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 2 * SEQ; ++c)
+                if (j0 + c < nby) row[c] = val[c];
+        }
+    }
+}
+
+// Dynamic LDS beyond 64 KiB per group needs the kernel's opt-in.
+hipError_t allow_lds(const void* kernel, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 }  // namespace
 
-void fbp_blas_destroy(void* blas) {
-    if (blas) (void)rocblas_destroy_handle((rocblas_handle)blas);
+int fbp_fft_log2(int nbx) {
+    int logl = 1;
+    while ((1 << logl) < 2 * nbx - 1) ++logl;
+    return logl;
 }
 
 hipError_t launch_fbp_filter(const DevGeom& g, const ViewTab* views, int a0, int a1,
-                             const float* taps, int ntaps, const float* p, float* pw,
-                             float* T_buf, float* q, void** blas, hipStream_t st) {
+                             const float* taps, int ntaps, const float* p, float2* spec,
+                             float* q, int lds_limit, hipStream_t st) {
     const int n_slots = a1 - a0;
     if (n_slots <= 0) return hipSuccess;
-    hipError_t e = launch_fbp_weight(g, views, a0, a1, p, pw, st);
+    const int logl = fbp_fft_log2(g.nbx);
+    const int L = 1 << logl;
+    float2* tw = spec;        // L/2
+    float2* hs = spec + L / 2;  // L
+    const size_t spec_lds = (size_t)(L + L / 2) * sizeof(float2);
+    const size_t lds2 = (size_t)(L / 2 + 2 * L) * sizeof(float2);
+    const size_t lds1 = (size_t)(L / 2 + L) * sizeof(float2);
+    if (lds1 > (size_t)lds_limit || spec_lds > (size_t)lds_limit) return hipErrorInvalidValue;
+    hipError_t e = allow_lds((const void*)fbp_spectrum_kernel, spec_lds);
     if (e != hipSuccess) return e;
-    const int64_t nt = (int64_t)g.nbx * g.nbx;
-    const int64_t blocks = (nt + 255) / 256 < 8192 ? (nt + 255) / 256 : 8192;
-    hipLaunchKernelGGL(toeplitz_kernel, dim3((unsigned)blocks), dim3(256), 0, st, taps, ntaps,
-                       g.nbx, T_buf);
+    hipLaunchKernelGGL(fbp_spectrum_kernel, dim3(1), dim3(FFT_THREADS), spec_lds, st, taps,
+                       ntaps, g.nbx, logl, tw, hs);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (!*blas) {
-        rocblas_handle hb = nullptr;
-        if (rocblas_create_handle(&hb) != rocblas_status_success) return hipErrorUnknown;
-        *blas = hb;
+    const int vec = (g.nby % 4 == 0) && ((uintptr_t)p % 16 == 0) && ((uintptr_t)q % 16 == 0);
+    if (lds2 <= (size_t)lds_limit) {
+        if ((e = allow_lds((const void*)fbp_filter_kernel<2>, lds2)) != hipSuccess) return e;
+        const int n_chunks = (g.nby + 3) / 4;
+        hipLaunchKernelGGL(fbp_filter_kernel<2>, dim3((unsigned)(n_chunks * n_slots)),
+                           dim3(FFT_THREADS), lds2, st, g, views, a0, n_slots, n_chunks, logl,
+                           tw, hs, p, q, vec);
+    } else {
+        if ((e = allow_lds((const void*)fbp_filter_kernel<1>, lds1)) != hipSuccess) return e;
+        const int n_chunks = (g.nby + 1) / 2;
+        hipLaunchKernelGGL(fbp_filter_kernel<1>, dim3((unsigned)(n_chunks * n_slots)),
+                           dim3(FFT_THREADS), lds1, st, g, views, a0, n_slots, n_chunks, logl,
+                           tw, hs, p, q, 0);
     }
-    rocblas_handle hb = (rocblas_handle)*blas;
-    if (rocblas_set_stream(hb, st) != rocblas_status_success) return hipErrorUnknown;
-    const float one = 1.0f, zero = 0.0f;
-    const rocblas_stride sv = (rocblas_stride)g.nbx * g.nby;
-    const rocblas_status s = rocblas_sgemm_strided_batched(
-        hb, rocblas_operation_none, rocblas_operation_none, g.nby, g.nbx, g.nbx, &one, pw,
-        g.nby, sv, T_buf, g.nbx, 0, &zero, q, g.nby, sv, n_slots);
-    return s == rocblas_status_success ? hipSuccess : hipErrorUnknown;
+    return hipGetLastError();
 }
 
 }  // namespace ostr

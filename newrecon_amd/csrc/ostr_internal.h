@@ -378,17 +378,15 @@ hipError_t launch_momentum_step(void* x_next, const void* z, const void* z_prev,
 hipError_t launch_add_f64(double* acc, const float* x, int64_t n, hipStream_t st);
 hipError_t launch_f64_to_f32(float* out, const double* acc, int64_t n, hipStream_t st);
 hipError_t launch_fill_f64(double* p, double v, int64_t n, hipStream_t st);
-// FDK-style cosine weighting of views (DBT FBP): pw = D / |S - E| . p.
-hipError_t launch_fbp_weight(const DevGeom& g, const ViewTab* views, int a0, int a1,
-                             const float* p, float* pw, hipStream_t st);
-// Cosine weighting, then the row filter along detector x as one GEMM per view
-// on the matrix cores (fbp.hip): q[a] = T pw[a], T the nbx x nbx Toeplitz
-// matrix of the centred taps (written to T_buf, nbx^2 floats). *blas holds
-// the library handle, created on first use (fbp_blas_destroy frees it).
+// Row filter of the DBT FBP (fbp.hip): cosine weight D / |S - E| and the
+// convolution along detector x with the centred taps, by FFT in LDS over
+// L = 2^fbp_fft_log2(nbx) >= 2 nbx - 1 points. spec: 3 L / 2 float2 of scratch
+// (twiddles and the taps' spectrum); lds_limit: the device's LDS bytes per
+// group (hipErrorInvalidValue when one column pair does not fit).
+int fbp_fft_log2(int nbx);
 hipError_t launch_fbp_filter(const DevGeom& g, const ViewTab* views, int a0, int a1,
-                             const float* taps, int ntaps, const float* p, float* pw,
-                             float* T_buf, float* q, void** blas, hipStream_t st);
-void fbp_blas_destroy(void* blas);
+                             const float* taps, int ntaps, const float* p, float2* spec,
+                             float* q, int lds_limit, hipStream_t st);
 hipError_t launch_line_integrals(float* p, const float* y, const float* b, int64_t n,
                                  hipStream_t st);
 // Fused OSTR subset update with the exact transpose (hsub detector-row-major;

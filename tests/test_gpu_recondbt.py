@@ -144,7 +144,7 @@ def test_fbp_matches_oracle(rdbt_case, dev, window):
     qr = F.fbp_filter(s.geom, s.angles, p, taps.astype(np.float32).astype(np.float64))
     err = float(np.abs(q.cpu().numpy() - qr).max() / np.abs(qr).max())
     assert err <= 1e-5, err
-    # the filter's GEMM is deterministic: a second call gives the same bits
+    # the FFT filter is deterministic: a second call gives the same bits
     q2 = S.fbp_filter(torch.from_numpy(p).to(dev),
                       torch.from_numpy(taps.astype(np.float32)).to(dev), 0, s.n_views)
     assert torch.equal(q, q2)
