@@ -37,7 +37,8 @@ int ostr_cnn_pack(const float* w, int32_t cin, int32_t cout, void* wf, void* wd,
  * at 0; mask != NULL zeroes outputs where mask <= 0 (the ReLU gradient). The
  * data gradient is this call on the output gradient with wd, cin/cout
  * swapped, bias NULL and the layer input as mask. Channel strides/offsets of
- * x multiples of 8, of y and mask multiples of 4. (cin, cout) must be one of
+ * x multiples of 8, of y and mask multiples of 4; x and bias 16-byte
+ * aligned, y and mask 8-byte aligned (OSTR_E_ARG otherwise). (cin, cout) must be one of
  * the compiled shapes (8,16) (16,16) (16,32) (32,32) (32,64) (64,64) (96,32)
  * (48,16) (32,16) (64,32) (32,96) (16,48). */
 int ostr_cnn_conv3d(int32_t cin, int32_t cout, const void* x, int32_t xcs, int32_t xco,

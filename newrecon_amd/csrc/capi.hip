@@ -1507,6 +1507,9 @@ OSTR_API int ostr_cnn_conv3d(int32_t cin, int32_t cout, const void* x, int32_t x
         (mask && (mcs % 4 || mco % 4 || mco + cout > mcs)))
         return cnn_fail(OSTR_E_ARG, "ostr_cnn_conv3d: channel stride/offset not aligned or "
                                     "out of range");
+    if ((uintptr_t)x % 16 || (uintptr_t)bias % 16 || (uintptr_t)y % 8 || (uintptr_t)mask % 8)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_conv3d: x and bias must be 16-byte, y and mask "
+                                    "8-byte aligned");
     return cnn_done(ostr::launch_cnn_conv(cin, cout, x, xcs, xco, wf, bias, y, ycs, yco, mask,
                                           mcs, mco, relu, nz, ny, nx, (hipStream_t)stream),
                     "ostr_cnn_conv3d");

@@ -15,10 +15,10 @@
 //   D[co][voxel] += sum_k W[co][k] * P[k][voxel],  k = tap * CIN + ci,
 //   tap = (dz+1)*9 + (dy+1)*3 + (dx+1),  P[k][v] = X[v + (dz,dy,dx)][ci].
 // A is read as 16 bytes of the packed weights, B as 16 bytes (8 channels of
-// one tap) of the input, and a lane stores 4 consecutive output channels of
-// one voxel (8 bytes). The data gradient is the same kernel on the output
-// gradient with the flipped, transposed weights, masked by the layer input's
-// ReLU (dX = 0 where X <= 0).
+// one tap) of the input from a ring of input slabs in LDS, and a lane stores 4
+// consecutive output channels of one voxel (8 bytes). The data gradient is the
+// same kernel on the output gradient with the flipped, transposed weights,
+// masked by the layer input's ReLU (dX = 0 where X <= 0).
 //
 // Weight gradient: dW[co][tap][ci] = sum_v dY[v][co] X[v + off(tap)][ci],
 // a GEMM reduced over voxels: A = dY (16 co x 32 voxels), B = X shifted by
@@ -53,73 +53,304 @@ __device__ __forceinline__ bf16x8 zero8() {
     return z;
 }
 
-// ---- convolution (forward and data gradient) -------------------------------
-// Work-group: 4 waves x NVW voxels along x of one row (y, z).
-template <int CIN, int COUT, int NVW>
-__global__ __launch_bounds__(256) void conv3d_kernel(
+// ---- convolution through an LDS ring of input slabs -----------------------
+// Work-group: 4 compute waves and 1-2 loader waves; tile XW voxels along x by 4R
+// rows along y, marching in z over the slices [z0, z0 + zc). The input slabs
+// of slices z - 1, z, z + 1 (x0 - 1 .. x0 + XW, y0 - 1 .. y0 + 4R, all CIN
+// channels, zero outside the volume) sit in a 3-slot LDS ring, slice s in
+// slot s mod 3, layout [c8][row][XW + 2] of 16-byte elements (8 channels): the
+// 16 lanes of one B operand read 256 contiguous bytes. While the compute
+// waves work on slice z, the loader waves hold slice z + 2's slab in
+// registers and writes it over slice z - 1's slot between two barriers, so
+// the compute waves never wait on the slab loads (a wave's memory counter is
+// in order: a wait for its own weight loads would also wait for them).
+// Each compute wave owns R output rows. The K sum is grouped by dy: chunk
+// (dy, q) holds the pairs p = 4 q + kg -> (dz, dx, c8) = (p / (3 NC8),
+// (p / NC8) % 3, p % NC8) (pairs past 9 NC8 get zero weights), so the B
+// operand of input row r and chunk q serves output rows r - dyi, dyi = dy + 1,
+// of all three dy: (R + 2) LDS reads per 3 R MFMAs per column block and
+// output-channel block. A operands are read from the packed weights
+// (tap * CIN + 8 c8, 16 bytes): all of them once, into registers, where they
+// fit, else one chunk ahead.
+// Slab elements per LDS slot, and loader waves: each holds at most 24 of its
+// 16-byte elements per lane in flight.
+constexpr int conv_slab(int cin, int r, int xw) { return cin / 8 * (4 * r + 2) * (xw + 2); }
+constexpr int conv_loaders(int slab) { return (slab + 64 * 24 - 1) / (64 * 24); }
+
+template <int CIN, int COUT, int R, int XW>
+__global__ __launch_bounds__(256 + 64 * conv_loaders(conv_slab(CIN, R, XW))) void conv3d_lds_kernel(
     const u16* __restrict__ x, int xcs, int xco, const u16* __restrict__ w,
     const float* __restrict__ bias, u16* __restrict__ y, int ycs, int yco,
-    const u16* __restrict__ mask, int mcs, int mco, int relu, int nz, int ny, int nx) {
-    constexpr int KP = kpad(CIN), NCHUNK = KP / 32, NCO = COUT / 16, NVB = NVW / 16;
+    const u16* __restrict__ mask, int mcs, int mco, int relu, int nz, int ny, int nx,
+    int tiles_x, int tiles_y, int zc) {
+    constexpr int NC8 = CIN / 8, NP = 9 * NC8, NQ = (NP + 3) / 4;
+    constexpr int KP = kpad(CIN), NCO = COUT / 16, NVB = XW / 16;
+    constexpr int TR = 4 * R + 2, TW = XW + 2, SLAB = conv_slab(CIN, R, XW);
+    constexpr int LW = conv_loaders(SLAB), NT = 256 + 64 * LW;
+    extern __shared__ uint4 slabs[];  // 3 * SLAB
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tiles_x = (nx + 4 * NVW - 1) / (4 * NVW);
-    int b = blockIdx.x;
-    const int tx = b % tiles_x;
-    b /= tiles_x;
-    const int iy = b % ny, iz = b / ny;
-    const int xw = tx * 4 * NVW + wave * NVW;
-    if (xw >= nx) return;  // whole wave past the row (no cross-wave sync)
-    const int j = lane & 15, kg = lane >> 4;
-    f32x4 acc[NCO][NVB];
+    const int n = lane & 15, kg = lane >> 4;
+    // group b runs on XCD b % 8: give each XCD a contiguous range of tiles so
+    // that neighbouring tiles, which share halo rows, meet in one L2
+    const int n_groups = (int)gridDim.x;
+    int t;
+    {
+        const int xcd = (int)blockIdx.x & 7, qq = n_groups >> 3, rr = n_groups & 7;
+        t = xcd * qq + (xcd < rr ? xcd : rr) + ((int)blockIdx.x >> 3);
+    }
+    const int tx = t % tiles_x;
+    t /= tiles_x;
+    const int ty = t % tiles_y, tz = t / tiles_y;
+    const int x0 = tx * XW, y0 = ty * 4 * R, z0 = tz * zc;
+    const int z1 = z0 + zc < nz ? z0 + zc : nz;
+
+    // slab element e -> (row, xi, c8), c8 fastest (contiguous global reads)
+    auto src_of = [&](int e, int zi, bool& ok) -> const uint4* {
+        const int c8 = e % NC8, rest = e / NC8;
+        const int xi = rest % TW, row = rest / TW;
+        const int gx = x0 - 1 + xi, gy = y0 - 1 + row;
+        ok = e < SLAB && (unsigned)zi < (unsigned)nz && (unsigned)gx < (unsigned)nx &&
+             (unsigned)gy < (unsigned)ny;
+        return reinterpret_cast<const uint4*>(x + (((int64_t)zi * ny + gy) * nx + gx) * xcs +
+                                              xco + c8 * 8);
+    };
+    auto dst_of = [&](int e, int zi) -> uint4* {
+        const int c8 = e % NC8, rest = e / NC8;
+        const int xi = rest % TW, row = rest / TW;
+        return slabs + ((zi + 3) % 3) * SLAB + (c8 * TR + row) * TW + xi;
+    };
+
+    // the first three slabs, by all waves
+    for (int zi = z0 - 1; zi <= z0 + 1; ++zi) {
+        constexpr int NA = (SLAB + NT - 1) / NT;
+        uint4 v[NA];
 #pragma unroll
-    for (int cb = 0; cb < NCO; ++cb)
-#pragma unroll
-        for (int nb = 0; nb < NVB; ++nb) acc[cb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int ch = 0; ch < NCHUNK; ++ch) {
-        const int k0 = ch * 32 + kg * 8;
-        const int tap = k0 / CIN, ci = k0 % CIN;  // 8 k of one tap (CIN % 8 == 0)
-        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        const int zz = iz + dz, yy = iy + dy;
-        const bool row_ok = tap < 27 && (unsigned)zz < (unsigned)nz && (unsigned)yy < (unsigned)ny;
-        bf16x8 bop[NVB];
-#pragma unroll
-        for (int nb = 0; nb < NVB; ++nb) {
-            const int xx = xw + nb * 16 + j + dx;
-            if (row_ok && (unsigned)xx < (unsigned)nx)
-                bop[nb] = load8(x + (((int64_t)zz * ny + yy) * nx + xx) * xcs + xco + ci);
-            else
-                bop[nb] = zero8();
+        for (int i = 0; i < NA; ++i) {
+            bool ok;
+            const uint4* src = src_of((int)threadIdx.x + NT * i, zi, ok);
+            v[i] = ok ? *src : make_uint4(0u, 0u, 0u, 0u);
         }
 #pragma unroll
-        for (int cb = 0; cb < NCO; ++cb) {
-            const bf16x8 aop = load8(w + (int64_t)(cb * 16 + j) * KP + k0);
-#pragma unroll
-            for (int nb = 0; nb < NVB; ++nb)
-                acc[cb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop, bop[nb], acc[cb][nb],
-                                                                      0, 0, 0);
+        for (int i = 0; i < NA; ++i) {
+            const int e = (int)threadIdx.x + NT * i;
+            if (e < SLAB) *dst_of(e, zi) = v[i];
         }
     }
+    __syncthreads();
+
+    if (wave >= 4) {
+        // loaders: slab z + 2 in registers during step z, stored between the barriers
+        constexpr int NL = (SLAB + 64 * LW - 1) / (64 * LW);
+        const int l0 = (int)threadIdx.x - 256;
+#pragma unroll 1
+        for (int z = z0; z + 1 < z1; ++z) {
+            uint4 v[NL];
 #pragma unroll
-    for (int nb = 0; nb < NVB; ++nb) {
-        const int xx = xw + nb * 16 + j;
-        if (xx >= nx) continue;
-        const int64_t v = ((int64_t)iz * ny + iy) * nx + xx;
-#pragma unroll
-        for (int cb = 0; cb < NCO; ++cb) {
-            const int c0 = cb * 16 + kg * 4;
-            u16 o[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float val = acc[cb][nb][r];
-                if (bias) val += bias[c0 + r];
-                if (relu) val = val > 0.0f ? val : 0.0f;
-                if (mask && !(from_bf16(mask[v * mcs + mco + c0 + r]) > 0.0f)) val = 0.0f;
-                o[r] = to_bf16(val);
+            for (int i = 0; i < NL; ++i) {
+                bool ok;
+                const uint4* src = src_of(l0 + 64 * LW * i, z + 2, ok);
+                v[i] = ok ? *src : make_uint4(0u, 0u, 0u, 0u);
             }
-            uint2 packed;
-            packed.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
-            packed.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
-            *reinterpret_cast<uint2*>(y + v * ycs + yco + c0) = packed;
+            __syncthreads();  // slice z - 1's slot is read no more
+#pragma unroll
+            for (int i = 0; i < NL; ++i) {
+                const int e = l0 + 64 * LW * i;
+                if (e < SLAB) *dst_of(e, z + 2) = v[i];
+            }
+            __syncthreads();
+        }
+        return;
+    }
+
+    auto load_a = [&](int q, bf16x8 (&a)[3][NCO]) {
+        const int p = 4 * q + kg;
+        const int c8 = p % NC8, dxi = (p / NC8) % 3, dzi = p / (3 * NC8);
+#pragma unroll
+        for (int dyi = 0; dyi < 3; ++dyi)
+#pragma unroll
+            for (int cb = 0; cb < NCO; ++cb)
+                a[dyi][cb] = p < NP ? load8(w + (int64_t)(cb * 16 + n) * KP +
+                                            (dzi * 9 + dyi * 3 + dxi) * CIN + c8 * 8)
+                                    : zero8();
+    };
+    // B operands: row r of chunk q is read from LDS two row steps before its
+    // MFMAs, into slot r of a ring of R + 2 row buffers (the reads of the next
+    // chunk's rows 0 and 1 overlap this chunk's last two rows); scheduling
+    // barriers keep the compiler from sinking the reads to their use.
+    constexpr int RR = R + 2;
+    constexpr bool PIPE = R == 1 && NCO == 1;
+    auto bptr = [&](int z, int q) -> const uint4* {
+        // pairs past the last read a valid slab element (their weights are 0)
+        const int p = 4 * q + kg < NP ? 4 * q + kg : NP - 1;
+        const int c8 = p % NC8, dxi = (p / NC8) % 3, dzi = p / (3 * NC8);
+        return slabs + ((z + dzi + 2) % 3) * SLAB + (c8 * TR + wave * R) * TW + n + dxi;
+    };
+    bf16x8 ring[PIPE ? RR : 1][NVB];
+    auto read_row = [&](const uint4* bp, int r, bf16x8 (&dst)[NVB]) {
+#pragma unroll
+        for (int nb = 0; nb < NVB; ++nb)
+            dst[nb] = *reinterpret_cast<const bf16x8*>(bp + r * TW + 16 * nb);
+    };
+    // one chunk: the B operand of input row r serves the output rows r - dyi.
+    // With one row and one channel block per wave (a few MFMAs per row) the
+    // reads run two row steps ahead through the ring; otherwise the compiler's
+    // own interleaving of reads and MFMAs measured faster.
+    auto chunk_direct = [&](int z, int q, const bf16x8 (&a)[3][NCO],
+                            f32x4 (&acc)[R][NVB][NCO]) {
+        const uint4* bp = bptr(z, q);
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            bf16x8 bop[NVB];
+            read_row(bp, r, bop);
+#pragma unroll
+            for (int dyi = 0; dyi < 3; ++dyi) {
+                const int o = r - dyi;
+                if (o < 0 || o >= R) continue;
+#pragma unroll
+                for (int nb = 0; nb < NVB; ++nb)
+#pragma unroll
+                    for (int cb = 0; cb < NCO; ++cb)
+                        acc[o][nb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            a[dyi][cb], bop[nb], acc[o][nb][cb], 0, 0, 0);
+            }
+        }
+    };
+    auto chunk_ring = [&](int z, int q, const uint4*& bq, const bf16x8 (&a)[3][NCO],
+                          f32x4 (&acc)[R][NVB][NCO]) {
+        const bool more_q = q + 1 < NQ;
+        const uint4* bn = bptr(z, more_q ? q + 1 : q);
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            if (r + 2 < RR)
+                read_row(bq, r + 2, ring[(r + 2) % (PIPE ? RR : 1)]);
+            else if (more_q)
+                read_row(bn, r + 2 - RR, ring[(r + 2 - RR) % (PIPE ? RR : 1)]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int dyi = 0; dyi < 3; ++dyi) {
+                const int o = r - dyi;
+                if (o < 0 || o >= R) continue;
+#pragma unroll
+                for (int nb = 0; nb < NVB; ++nb)
+#pragma unroll
+                    for (int cb = 0; cb < NCO; ++cb)
+                        acc[o][nb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            a[dyi][cb], ring[r % (PIPE ? RR : 1)][nb], acc[o][nb][cb], 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        bq = bn;
+    };
+    auto chunk = [&](int z, int q, const uint4*& bq, const bf16x8 (&a)[3][NCO],
+                     f32x4 (&acc)[R][NVB][NCO]) {
+        if constexpr (PIPE)
+            chunk_ring(z, q, bq, a, acc);
+        else
+            chunk_direct(z, q, a, acc);
+    };
+    // VGPR budget (256 per lane: a group of 5-6 waves puts two on some SIMD).
+    // The weights stay in registers for the whole kernel when they, the
+    // accumulators and the row ring fit (in 220 VGPRs; 180 where the compiler
+    // schedules the reads); otherwise each chunk's are loaded one chunk ahead,
+    // or at the chunk where two chunks' would not fit.
+    constexpr int ACC = 4 * R * NVB * NCO + (PIPE ? 4 * RR * NVB : 0);
+    constexpr bool AREG = 12 * NQ * NCO + ACC <= (PIPE ? 220 : 180);
+    constexpr int NQR = AREG ? NQ : 1, NQA = AREG || 24 * NCO + ACC > 200 ? 1 : 2;
+    bf16x8 aw[NQR][3][NCO], ab[NQA][3][NCO];
+    if constexpr (AREG) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) load_a(q, aw[q]);
+    } else if constexpr (NQA == 2) {
+        load_a(0, ab[0]);
+    }
+#pragma unroll 1
+    for (int z = z0; z < z1; ++z) {
+        f32x4 acc[R][NVB][NCO];
+#pragma unroll
+        for (int o = 0; o < R; ++o)
+#pragma unroll
+            for (int nb = 0; nb < NVB; ++nb)
+#pragma unroll
+                for (int cb = 0; cb < NCO; ++cb) acc[o][nb][cb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        const uint4* bq = bptr(z, 0);
+        if constexpr (PIPE) {
+            read_row(bq, 0, ring[0]);
+            read_row(bq, 1, ring[1]);
+        }
+        if constexpr (AREG) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) chunk(z, q, bq, aw[q], acc);
+        } else {
+#pragma unroll 1
+            for (int q = 0; q < NQ; ++q) {
+                if constexpr (NQA == 2) {
+                    // after the last chunk: the next slice's first
+                    load_a(q + 1 < NQ ? q + 1 : 0, ab[NQA - 1]);
+                    chunk(z, q, bq, ab[0], acc);
+#pragma unroll
+                    for (int dyi = 0; dyi < 3; ++dyi)
+#pragma unroll
+                        for (int cb = 0; cb < NCO; ++cb) ab[0][dyi][cb] = ab[NQA - 1][dyi][cb];
+                } else {
+                    load_a(q, ab[0]);
+                    chunk(z, q, bq, ab[0], acc);
+                }
+            }
+        }
+        // epilogue: the bias and the ReLU masks of all outputs first (4 channels
+        // = 8 bytes), then the stores
+        float4 bv[NCO];
+#pragma unroll
+        for (int cb = 0; cb < NCO; ++cb)
+            bv[cb] = bias ? *reinterpret_cast<const float4*>(bias + cb * 16 + kg * 4)
+                          : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint2 mk[R][NVB][NCO];
+#pragma unroll
+        for (int o = 0; o < R; ++o)
+#pragma unroll
+            for (int nb = 0; nb < NVB; ++nb)
+#pragma unroll
+                for (int cb = 0; cb < NCO; ++cb) {
+                    const int gy = y0 + wave * R + o, gx = x0 + nb * 16 + n;
+                    const int64_t v = ((int64_t)z * ny + gy) * nx + gx;
+                    mk[o][nb][cb] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+                    if (mask && gy < ny && gx < nx)
+                        mk[o][nb][cb] = *reinterpret_cast<const uint2*>(
+                            mask + v * mcs + mco + cb * 16 + kg * 4);
+                }
+#pragma unroll
+        for (int o = 0; o < R; ++o) {
+            const int gy = y0 + wave * R + o;
+            if (gy >= ny) continue;
+#pragma unroll
+            for (int nb = 0; nb < NVB; ++nb) {
+                const int gx = x0 + nb * 16 + n;
+                if (gx >= nx) continue;
+                const int64_t v = ((int64_t)z * ny + gy) * nx + gx;
+#pragma unroll
+                for (int cb = 0; cb < NCO; ++cb) {
+                    const int c0 = cb * 16 + kg * 4;
+                    const uint32_t m2[2] = {mk[o][nb][cb].x, mk[o][nb][cb].y};
+                    u16 ov[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        float val = acc[o][nb][cb][e];
+                        if (bias) val += e == 0 ? bv[cb].x : e == 1 ? bv[cb].y : e == 2 ? bv[cb].z : bv[cb].w;
+                        if (relu) val = val > 0.0f ? val : 0.0f;
+                        const u16 mb = (u16)(m2[e >> 1] >> (16 * (e & 1)));
+                        if (mask && !(from_bf16(mb) > 0.0f)) val = 0.0f;
+                        ov[e] = to_bf16(val);
+                    }
+                    uint2 packed;
+                    packed.x = (uint32_t)ov[0] | ((uint32_t)ov[1] << 16);
+                    packed.y = (uint32_t)ov[2] | ((uint32_t)ov[3] << 16);
+                    *reinterpret_cast<uint2*>(y + v * ycs + yco + c0) = packed;
+                }
+            }
+        }
+        if (z + 1 < z1) {
+            __syncthreads();  // the loader overwrites slice z - 1's slot
+            __syncthreads();  // slice z + 2 is in place
         }
     }
 }
@@ -531,15 +762,38 @@ unsigned grid_for(int64_t n, int per = 256) {
     return (unsigned)(g < 1 ? 1 : g);
 }
 
+// Tile of the LDS convolution: R output rows per wave, XW voxels along x.
+template <int CIN, int COUT>
+struct ConvTile {
+    static constexpr int R = CIN <= 8 ? 4 : (CIN <= 16 ? 2 : 1);
+    static constexpr int XW = CIN <= 16 ? 64 : 32;
+    static constexpr size_t LDS = (size_t)3 * (CIN / 8) * (4 * R + 2) * (XW + 2) * 16;
+};
+
 template <int CIN, int COUT>
 hipError_t conv_t(const u16* x, int xcs, int xco, const u16* w, const float* bias, u16* y,
                   int ycs, int yco, const u16* mask, int mcs, int mco, int relu, int nz, int ny,
                   int nx, hipStream_t st) {
-    constexpr int NVW = COUT >= 64 ? 16 : 32;
-    const int64_t tiles_x = (nx + 4 * NVW - 1) / (4 * NVW);
-    const int64_t g = tiles_x * ny * nz;
-    hipLaunchKernelGGL((conv3d_kernel<CIN, COUT, NVW>), dim3((unsigned)g), dim3(256), 0, st, x,
-                       xcs, xco, w, bias, y, ycs, yco, mask, mcs, mco, relu, nz, ny, nx);
+    using T = ConvTile<CIN, COUT>;
+    const auto kern = conv3d_lds_kernel<CIN, COUT, T::R, T::XW>;
+    if (T::LDS > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void*)kern,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)T::LDS);
+        if (e != hipSuccess) return e;
+    }
+    const int tiles_x = (nx + T::XW - 1) / T::XW, tiles_y = (ny + 4 * T::R - 1) / (4 * T::R);
+    // about 256 groups (one per CU): z chunks of zc slices, each with a
+    // three-slab prologue; longer chunks measured faster at the training size
+    const int64_t tiles = (int64_t)tiles_x * tiles_y;
+    int64_t nzc = (256 + tiles - 1) / tiles;
+    if (nzc > nz) nzc = nz;
+    const int zc = (int)((nz + nzc - 1) / nzc);
+    nzc = (nz + zc - 1) / zc;
+    const int nt = 256 + 64 * conv_loaders(conv_slab(CIN, T::R, T::XW));
+    hipLaunchKernelGGL(kern, dim3((unsigned)(tiles * nzc)), dim3(nt), T::LDS, st, x, xcs,
+                       xco, w, bias, y, ycs, yco, mask, mcs, mco, relu, nz, ny, nx, tiles_x,
+                       tiles_y, zc);
     return hipGetLastError();
 }
 

@@ -77,6 +77,51 @@ def test_conv3d_matches_fp64(dev, cin, cout):
     assert rel_l2(from_dev_bf16(dx).reshape(nz, ny, nx, cin), ref_dx) <= 1e-2
 
 
+@pytest.mark.parametrize("cin,cout", [(16, 16), (48, 16), (16, 48), (32, 32), (96, 32)])
+def test_conv3d_z_march_matches_fp64(dev, cin, cout):
+    """11 x 101 x 250 voxels: each work-group marches over several slices
+    (its loader waves refill the LDS ring between them), and the last tiles
+    in x, y and z are partial. x and y are channel views of wider buffers
+    (offsets 8 and 4); a second call adds the ReLU mask of the data gradient
+    (mask a channel view too) and no bias. Bar: relative L2 <= 1e-2 against
+    the FP64 convolution of the same BF16 inputs, and the same bits on a repeat."""
+    net = W.WarmStartNet(dev)
+    L = net._L
+    rng = np.random.default_rng(cin * 7 + cout)
+    nz, ny, nx = 11, 101, 250
+    xcs, xco, ycs, yco = cin + 16, 8, cout + 8, 4
+    x = bf16_round(rng.standard_normal((nz, ny, nx, cin)))
+    w = bf16_round(rng.standard_normal((cout, 27, cin)) / np.sqrt(27 * cin))
+    b = rng.standard_normal(cout).astype(np.float32)
+    wf = torch.zeros(cout * L.ostr_cnn_kpad(cin), dtype=torch.int16, device=dev)
+    wd = torch.zeros(cin * L.ostr_cnn_kpad(cout), dtype=torch.int16, device=dev)
+    wt = torch.from_numpy(w.astype(np.float32)).to(dev)
+    net._check(L.ostr_cnn_pack(W._ptr(wt), cin, cout, W._ptr(wf), W._ptr(wd), None))
+    xbuf = np.zeros((nz, ny, nx, xcs))
+    xbuf[..., xco:xco + cin] = x
+    xd = to_dev_bf16(xbuf.reshape(-1), dev)
+    y = torch.zeros(nz * ny * nx * ycs, dtype=torch.int16, device=dev)
+    bd = torch.from_numpy(b).to(dev)
+    ref = C.conv(x, w, b.astype(np.float64))
+    m = bf16_round(rng.standard_normal((nz, ny, nx, ycs)))
+    md = to_dev_bf16(m.reshape(-1), dev)
+    cases = ((W._ptr(bd), None, 1, np.maximum(ref, 0.0)),
+             (None, W._ptr(md), 0, np.where(m[..., yco:yco + cout] > 0, ref - b, 0.0)))
+    for bias, mask, relu, want in cases:
+        outs = []
+        for _ in range(2):
+            y.zero_()
+            net._check(L.ostr_cnn_conv3d(cin, cout, W._ptr(xd), xcs, xco, W._ptr(wf), bias,
+                                         W._ptr(y), ycs, yco, mask, ycs, yco, relu, nz, ny, nx,
+                                         None))
+            torch.cuda.synchronize()
+            outs.append(y.clone())
+        assert torch.equal(outs[0], outs[1])
+        got = from_dev_bf16(y).reshape(nz, ny, nx, ycs)
+        assert not got[..., :yco].any() and not got[..., yco + cout:].any()
+        assert rel_l2(got[..., yco:yco + cout], want) <= 1e-2
+
+
 def oracle_params_from(net):
     return {k: (bf16_round(w), b.astype(np.float64)) for k, (w, b) in net.host_params().items()}
 
