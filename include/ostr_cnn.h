@@ -52,7 +52,11 @@ size_t ostr_cnn_wgrad_workspace(int32_t cin, int32_t cout, int32_t nz, int32_t n
 
 /* dw [cout][27][cin] = gradient of the convolution weights for input x and
  * output gradient dy (FP32, written); db [cout] likewise (may be NULL).
- * Deterministic: nsplit (1..1024) partial sums added in a fixed order. */
+ * Deterministic: partial sums added in a fixed order. The shapes (8,16),
+ * (16,16) and (48,16) run from LDS (one partial per work-group of a fixed
+ * grid; nsplit unused) and need channel strides/offsets that are multiples of
+ * 8 and 16-byte aligned x and dy (OSTR_E_ARG otherwise); the others reduce
+ * nsplit (1..1024) row shares from channel-major copies in the workspace. */
 int ostr_cnn_wgrad(int32_t cin, int32_t cout, const void* x, int32_t xcs, int32_t xco,
                    const void* dy, int32_t dcs, int32_t dco, int32_t nz, int32_t ny, int32_t nx,
                    float* dw, float* db, void* work, size_t work_bytes, int32_t nsplit,

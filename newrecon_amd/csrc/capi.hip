@@ -1531,6 +1531,11 @@ OSTR_API int ostr_cnn_wgrad(int32_t cin, int32_t cout, const void* x, int32_t xc
         return cnn_fail(OSTR_E_ARG, "ostr_cnn_wgrad: bad argument");
     if (xco + cin > xcs || dco + cout > dcs)
         return cnn_fail(OSTR_E_ARG, "ostr_cnn_wgrad: channel view out of range");
+    if (ostr::cnn_wgrad_lds_shape(cin, cout) &&
+        (xcs % 8 || xco % 8 || dcs % 8 || dco % 8 || (uintptr_t)x % 16 || (uintptr_t)dy % 16))
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_wgrad: %dx%d needs channel strides/offsets that "
+                                    "are multiples of 8 and 16-byte aligned x and dy",
+                        cin, cout);
     if (work_bytes < ostr::cnn_wgrad_work_bytes(cin, cout, nz, ny, nx, nsplit))
         return cnn_fail(OSTR_E_RANGE, "ostr_cnn_wgrad: workspace too small");
     return cnn_done(ostr::launch_cnn_wgrad(cin, cout, x, xcs, xco, dy, dcs, dco, nz, ny, nx, dw,

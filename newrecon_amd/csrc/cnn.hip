@@ -429,6 +429,193 @@ __global__ __launch_bounds__(256) void to_cm_tile_kernel(const u16* __restrict__
     }
 }
 
+// ---- weight gradient through an LDS ring (COUT = 16) ----------------------
+// dW[co][tap][ci] = sum_v dY[v][co] X[v + off(tap)][ci] and db[co] =
+// sum_v dY[v][co] reduce over voxels, so both MFMA operands hold 8
+// consecutive voxels of one channel (K = 32 voxels of one row half). They
+// are read from channels-last LDS slabs with the transposing read
+// ds_read_b64_tr_b16: within each 16 lanes, lane i receives element i & 3 of
+// the 4 that source lanes (i >> 2) + 4 e (e = 0..3) address, so when source
+// lane s addresses channels 4 (s & 3) .. + 3 of voxel v0 + (s >> 2), lane i
+// receives voxels v0 .. v0 + 3 of channel i (two reads: 8 voxels); any voxel
+// shift works, so the 27 taps need no shifted copies. Work-group: a 64 x 8
+// (x, y) tile (64 x 4 for 48 input channels) marching in z over [z0, z0 + zc) with the X slabs of slices
+// z - 1 .. z + 1 (halo included, zero outside the volume) in a 3-slot ring and
+// dY of slice z in a 2-slot ring, both in the convolution's [c8][row][x]
+// layout, the next slices fetched into registers during compute (no global
+// load is waited on inside the step); the 27 NCI (tap,
+// 16-channel block) accumulators are dealt to the 4 compute waves round-robin
+// (wave 0 also sums dY against ones for the bias). Each group writes its
+// partial sums: part[g][co][tap CIN + ci], bpart[g][co]. Channels past CIN
+// (CIN = 8) read channel 0..7 again; their columns are not written.
+// x by y tile: 64 x 8 voxels for up to 16 input channels, 64 x 4 for 48
+constexpr int WG_XW = 64;
+constexpr int wgrad_ty(int cin) { return cin <= 16 ? 8 : 4; }
+constexpr int wgrad_slab(int cin) { return cin / 8 * (wgrad_ty(cin) + 2) * (WG_XW + 2); }
+constexpr int wgrad_dslab(int cin) { return 2 * wgrad_ty(cin) * WG_XW; }
+
+typedef short v4i16 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ bf16x8 tr_read8(const uint4* base, int off0, int off1) {
+    typedef __attribute__((address_space(3))) v4i16 lds_v4;
+    const char* b = reinterpret_cast<const char*>(base);
+    typedef short v8i16 __attribute__((ext_vector_type(8)));
+    const v4i16 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4*)(b + off0));
+    const v4i16 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4*)(b + off1));
+    const v8i16 both = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_bit_cast(bf16x8, both);
+}
+
+template <int CIN>
+__global__ __launch_bounds__(256) void wgrad_lds_kernel(
+    const u16* __restrict__ x, int xcs, int xco, const u16* __restrict__ dy, int dcs, int dco,
+    float* __restrict__ part, float* __restrict__ bpart, int nz, int ny, int nx, int tiles_x,
+    int tiles_y, int zc) {
+    constexpr int COUT = 16, NC8 = CIN / 8, NCI = (CIN + 15) / 16, NTILE = 27 * NCI;
+    constexpr int WG_TY = wgrad_ty(CIN), TR = WG_TY + 2, TW = WG_XW + 2;
+    constexpr int SLAB = wgrad_slab(CIN), DSLAB = wgrad_dslab(CIN);
+    constexpr int NTW = (NTILE + 3) / 4;  // accumulator tiles per wave
+    extern __shared__ uint4 lds[];      // 3 X slabs, then 2 dY slabs
+    uint4* xs = lds;
+    uint4* ds = lds + 3 * SLAB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n_groups = (int)gridDim.x;
+    int t;
+    {
+        const int xcd = (int)blockIdx.x & 7, qq = n_groups >> 3, rr = n_groups & 7;
+        t = xcd * qq + (xcd < rr ? xcd : rr) + ((int)blockIdx.x >> 3);
+    }
+    const int group = t;
+    const int tx = t % tiles_x;
+    t /= tiles_x;
+    const int ty = t % tiles_y, tz = t / tiles_y;
+    const int x0 = tx * WG_XW, y0 = ty * WG_TY, z0 = tz * zc;
+    const int z1 = z0 + zc < nz ? z0 + zc : nz;
+
+    // This thread's slab elements e = threadIdx.x + 256 i: X slab elements
+    // (e < SLAB, [c8][row][x] with the halo) and dY slab elements (SLAB <= e <
+    // SLAB + DSLAB); their offsets inside a slice (-1: outside the volume's x/y
+    // range, zero) and their LDS index inside a slot, computed once. The next
+    // slabs are fetched into registers while the current slice is computed.
+    constexpr int NE = (SLAB + DSLAB + 255) / 256;
+    int soff[NE], loff[NE];
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+        const int e = (int)threadIdx.x + 256 * i;
+        soff[i] = -1;
+        loff[i] = 0;
+        if (e < SLAB) {
+            const int c8 = e % NC8, rest = e / NC8;
+            const int xi = rest % TW, row = rest / TW;
+            const int gx = x0 - 1 + xi, gy = y0 - 1 + row;
+            if ((unsigned)gx < (unsigned)nx && (unsigned)gy < (unsigned)ny)
+                soff[i] = (gy * nx + gx) * xcs + xco + c8 * 8;
+            loff[i] = (c8 * TR + row) * TW + xi;
+        } else if (e < SLAB + DSLAB) {
+            const int f = e - SLAB;
+            const int c8 = f & 1, rest = f >> 1;
+            const int xi = rest % WG_XW, row = rest / WG_XW;
+            const int gx = x0 + xi, gy = y0 + row;
+            if (gx < nx && gy < ny) soff[i] = (gy * nx + gx) * dcs + dco + c8 * 8;
+            loff[i] = (c8 * WG_TY + row) * WG_XW + xi;
+        }
+    }
+    const int64_t xslice = (int64_t)ny * nx * xcs, dslice = (int64_t)ny * nx * dcs;
+    uint4 pre[NE];
+    // X slab of slice zx; with dY (zd >= 0) also the dY of slice zd
+    auto fetch = [&](int zx, int zd) {
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            const int e = (int)threadIdx.x + 256 * i;
+            pre[i] = make_uint4(0u, 0u, 0u, 0u);
+            if (e < SLAB) {
+                if (soff[i] >= 0 && (unsigned)zx < (unsigned)nz)
+                    pre[i] = *reinterpret_cast<const uint4*>(x + zx * xslice + soff[i]);
+            } else if (e < SLAB + DSLAB) {
+                if (soff[i] >= 0 && zd >= 0 && zd < nz)
+                    pre[i] = *reinterpret_cast<const uint4*>(dy + zd * dslice + soff[i]);
+            }
+        }
+    };
+    auto store = [&](int zx, int zd) {
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            const int e = (int)threadIdx.x + 256 * i;
+            if (e < SLAB)
+                xs[((zx + 3) % 3) * SLAB + loff[i]] = pre[i];
+            else if (e < SLAB + DSLAB && zd >= 0)
+                ds[((zd + 2) % 2) * DSLAB + loff[i]] = pre[i];
+        }
+    };
+    fetch(z0 - 1, -1);
+    store(z0 - 1, -1);
+    fetch(z0, -1);
+    store(z0, -1);
+    fetch(z0 + 1, z0);
+    store(z0 + 1, z0);
+    __syncthreads();
+
+    // per-lane parts of the transposing reads: source lane s = lane & 15 of
+    // group kg addresses channels 4 (s & 3) .. + 3 of voxel 8 kg + (s >> 2)
+    const int s = lane & 15, kg = lane >> 4;
+    const int pc = (s & 3) >> 1, hb = (s & 1) * 8, vx = 8 * kg + (s >> 2);
+    f32x4 acc[NTW], accb = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < NTW; ++i) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x8 ones;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.0f;
+#pragma unroll 1
+    for (int z = z0; z < z1; ++z) {
+        const bool more = z + 1 < z1;
+        if (more) fetch(z + 2, z + 1);
+        const uint4* dslot = ds + ((z + 2) % 2) * DSLAB;
+#pragma unroll 1
+        for (int kc = 0; kc < 2 * WG_TY; ++kc) {
+            const int r = kc >> 1, xh = (kc & 1) * 32;
+            // A: dY of voxels (x0 + xh + 8 kg .. + 7, row r), channels 0..15
+            const int aoff = ((pc * WG_TY + r) * WG_XW + xh + vx) * 16 + hb;
+            const bf16x8 a = tr_read8(dslot, aoff, aoff + 4 * 16);
+            // branch-free, so that the reads of all tiles can be issued ahead:
+            // every wave sums the bias (wave 0's is kept), and a wave's slots
+            // past the last tile repeat it (not written)
+            accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ones, accb, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < NTW; ++i) {
+                const int u = wave + 4 * i < NTILE ? wave + 4 * i : NTILE - 1;
+                const int tap = u / NCI, cb = u % NCI;
+                const int dzi = tap / 9, dyi = (tap / 3) % 3, dxi = tap % 3;
+                const int c8 = 2 * cb + pc < NC8 ? 2 * cb + pc : NC8 - 1;
+                const uint4* xslot = xs + ((z + dzi + 2) % 3) * SLAB;
+                const int boff = ((c8 * TR + r + dyi) * TW + xh + vx + dxi) * 16 + hb;
+                const bf16x8 b = tr_read8(xslot, boff, boff + 4 * 16);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+            }
+        }
+        if (more) {
+            __syncthreads();  // slice z - 1's X slot and dY slot are read no more
+            store(z + 2, z + 1);
+            __syncthreads();
+        }
+    }
+    // D row co = 4 kg + e, column ci = lane & 15
+    constexpr int K = 27 * CIN;
+    float* out = part + (int64_t)group * COUT * K;
+#pragma unroll
+    for (int i = 0; i < NTW; ++i) {
+        const int u = wave + 4 * i;
+        if (u >= NTILE) break;
+        const int tap = u / NCI, ci = (u % NCI) * 16 + (lane & 15);
+        if (ci < CIN) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) out[(int64_t)(4 * kg + e) * K + tap * CIN + ci] = acc[i][e];
+        }
+    }
+    if (wave == 0 && (lane & 15) == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bpart[(int64_t)group * COUT + 4 * kg + e] = accb[e];
+    }
+}
+
 // Partial weight gradient over the rows of split blockIdx.x: part[split][co]
 // [tap * CIN + ci]. dy_cm: [COUT16][rows][nxp]; xs_cm: 3 shifted copies
 // [3][CIN16][rows][nxp] (copy s holds dx = s - 1). TX = 1: one tap per
@@ -850,16 +1037,77 @@ bool cnn_wgrad_supported(int cin, int cout) {
 
 int cnn_nxp(int nx) { return (nx + 31) / 32 * 32; }
 
+// The LDS-ring weight gradient (wgrad_lds_kernel): shapes, grid and workspace.
+bool cnn_wgrad_lds_shape(int cin, int cout) {
+    return cout == 16 && (cin == 8 || cin == 16 || cin == 48);
+}
+static bool wgrad_lds_shape(int cin, int cout) { return cnn_wgrad_lds_shape(cin, cout); }
+struct WgradGrid {
+    int tiles_x, tiles_y, zc, groups;
+};
+static WgradGrid wgrad_lds_grid(int cin, int nz, int ny, int nx) {
+    WgradGrid g;
+    g.tiles_x = (nx + WG_XW - 1) / WG_XW;
+    g.tiles_y = (ny + wgrad_ty(cin) - 1) / wgrad_ty(cin);
+    // about 256 groups (one per CU), each marching over zc slices
+    const int64_t tiles = (int64_t)g.tiles_x * g.tiles_y;
+    int64_t nzc = (256 + tiles - 1) / tiles;
+    if (nzc > nz) nzc = nz;
+    g.zc = (int)((nz + nzc - 1) / nzc);
+    g.groups = (int)(tiles * ((nz + g.zc - 1) / g.zc));
+    return g;
+}
+static size_t wgrad_lds_bytes(int cin, int cout, int nz, int ny, int nx) {
+    const WgradGrid g = wgrad_lds_grid(cin, nz, ny, nx);
+    return sizeof(float) * (size_t)g.groups * ((size_t)cout * 27 * cin + cout) + 256;
+}
+
 size_t cnn_wgrad_work_bytes(int cin, int cout, int nz, int ny, int nx, int nsplit) {
     const int64_t plane = (int64_t)nz * ny * cnn_nxp(nx);
     const int64_t ci16 = (cin + 15) / 16 * 16;
-    return sizeof(u16) * (size_t)(plane * cout + 3 * plane * ci16) +
-           sizeof(float) * ((size_t)nsplit * cout * 27 * cin + (size_t)BIAS_SPLIT * cout) + 256;
+    const size_t generic =
+        sizeof(u16) * (size_t)(plane * cout + 3 * plane * ci16) +
+        sizeof(float) * ((size_t)nsplit * cout * 27 * cin + (size_t)BIAS_SPLIT * cout) + 256;
+    return wgrad_lds_shape(cin, cout) ? wgrad_lds_bytes(cin, cout, nz, ny, nx) : generic;
+}
+
+template <int CIN>
+static hipError_t wgrad_lds_t(const void* x, int xcs, int xco, const void* dy, int dcs, int dco,
+                              int nz, int ny, int nx, float* dw, float* db, void* work,
+                              hipStream_t st) {
+    const WgradGrid g = wgrad_lds_grid(CIN, nz, ny, nx);
+    const int64_t n = (int64_t)16 * 27 * CIN;
+    float* part = (float*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
+    float* bpart = part + (int64_t)g.groups * n;
+    const size_t lds = (size_t)(3 * wgrad_slab(CIN) + 2 * wgrad_dslab(CIN)) * 16;
+    const auto kern = wgrad_lds_kernel<CIN>;
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(
+            (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)g.groups), dim3(256), lds, st, (const u16*)x, xcs,
+                       xco, (const u16*)dy, dcs, dco, part, bpart, nz, ny, nx, g.tiles_x,
+                       g.tiles_y, g.zc);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const float*)part,
+                       g.groups, n, dw);
+    if (db)
+        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, st, (const float*)bpart,
+                           g.groups, (int64_t)16, db);
+    return hipGetLastError();
 }
 
 hipError_t launch_cnn_wgrad(int cin, int cout, const void* x, int xcs, int xco, const void* dy,
                             int dcs, int dco, int nz, int ny, int nx, float* dw, float* db,
                             void* work, int nsplit, hipStream_t st) {
+    if (wgrad_lds_shape(cin, cout)) {
+        if (xcs % 8 || xco % 8 || dcs % 8 || dco % 8 || (uintptr_t)x % 16 || (uintptr_t)dy % 16)
+            return hipErrorInvalidValue;
+        if (cin == 8) return wgrad_lds_t<8>(x, xcs, xco, dy, dcs, dco, nz, ny, nx, dw, db, work, st);
+        if (cin == 16)
+            return wgrad_lds_t<16>(x, xcs, xco, dy, dcs, dco, nz, ny, nx, dw, db, work, st);
+        return wgrad_lds_t<48>(x, xcs, xco, dy, dcs, dco, nz, ny, nx, dw, db, work, st);
+    }
     const int nxp = cnn_nxp(nx);
     const int64_t plane = (int64_t)nz * ny * nxp;
     const int ci16 = (cin + 15) / 16 * 16;

@@ -317,6 +317,8 @@ hipError_t launch_batch_update(const BatchCase* cases, const int* sched, int n_c
 // stride, channel offset).
 bool cnn_conv_supported(int cin, int cout);
 bool cnn_wgrad_supported(int cin, int cout);
+// (cin, cout) whose weight gradient runs through LDS (views and pointers aligned)
+bool cnn_wgrad_lds_shape(int cin, int cout);
 int cnn_kpad(int c);
 int cnn_nxp(int nx);
 size_t cnn_wgrad_work_bytes(int cin, int cout, int nz, int ny, int nx, int nsplit);

@@ -122,6 +122,45 @@ def test_conv3d_z_march_matches_fp64(dev, cin, cout):
         assert rel_l2(got[..., yco:yco + cout], want) <= 1e-2
 
 
+@pytest.mark.parametrize("cin", [8, 16, 48])
+def test_weight_gradient_z_march_matches_fp64(dev, cin):
+    """The weight and bias gradients of the full-resolution shapes (cin -> 16)
+    at 13 x 37 x 150 voxels: work-groups march over 1-2 slices, the last x, y
+    and z tiles are partial, x and dY are channel views (offset 8 in wider
+    buffers). The products of BF16 values are exact in FP32, so the only
+    error is the FP32 accumulation over 72k voxels. Bar: relative L2 <= 1e-4
+    (weights and bias) against FP64, and the same bits on a repeat."""
+    net = W.WarmStartNet(dev)
+    L = net._L
+    rng = np.random.default_rng(40 + cin)
+    nz, ny, nx, cout = 13, 37, 150, 16
+    xcs, xco, dcs, dco = cin + 8, 8, 32, 8
+    x = bf16_round(rng.standard_normal((nz, ny, nx, cin)))
+    gy = bf16_round(rng.standard_normal((nz, ny, nx, cout)))
+    w = np.zeros((cout, 27, cin))
+    _, ref_dw, ref_db = C.conv_grads(x, w, gy)
+    xbuf = np.zeros((nz, ny, nx, xcs))
+    xbuf[..., xco:xco + cin] = x
+    dbuf = np.zeros((nz, ny, nx, dcs))
+    dbuf[..., dco:dco + cout] = gy
+    xd = to_dev_bf16(xbuf.reshape(-1), dev)
+    gd = to_dev_bf16(dbuf.reshape(-1), dev)
+    nbytes = L.ostr_cnn_wgrad_workspace(cin, cout, nz, ny, nx, 64)
+    work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    outs = []
+    for _ in range(2):
+        dw = torch.full((cout * 27 * cin,), float("nan"), device=dev)
+        db = torch.full((cout,), float("nan"), device=dev)
+        net._check(L.ostr_cnn_wgrad(cin, cout, W._ptr(xd), xcs, xco, W._ptr(gd), dcs, dco, nz, ny,
+                                    nx, W._ptr(dw), W._ptr(db), W._ptr(work), nbytes, 64, None))
+        torch.cuda.synchronize()
+        outs.append((dw.clone(), db.clone()))
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    dw, db = outs[0]
+    assert rel_l2(dw.cpu().numpy().reshape(cout, 27, cin), ref_dw) <= 1e-4
+    assert rel_l2(db.cpu().numpy(), ref_db) <= 1e-4
+
+
 def oracle_params_from(net):
     return {k: (bf16_round(w), b.astype(np.float64)) for k, (w, b) in net.host_params().items()}
 
