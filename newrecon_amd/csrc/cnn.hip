@@ -28,6 +28,8 @@
 // rows for one tap; its waves' sums are combined in LDS in a fixed order and
 // written as a partial; a second kernel sums the partials in a fixed order,
 // so the gradient is deterministic.
+#include <initializer_list>
+
 #include "ostr_internal.h"
 
 namespace ostr {
@@ -721,16 +723,23 @@ __global__ __launch_bounds__(256) void wgrad_kernel(const u16* __restrict__ dy_c
     }
 }
 
-// dw[i] = sum_s part[s][i] (s ascending), i < n
-__global__ void sum_parts_kernel(const float* __restrict__ part, int nsplit, int64_t n,
-                                 float* __restrict__ dw) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        float s = 0.0f;
-        for (int k = 0; k < nsplit; ++k) s += part[(int64_t)k * n + i];
-        dw[i] = s;
-    }
+// dw[i] = sum_s part[s][i], i < n, in a fixed order: four lanes per output
+// sum s = g, g + 4, ... (g = 0..3, s ascending), then (((g0 + g1) + g2) + g3).
+// A work-group covers 64 outputs (grid: ceil(n / 64) groups of 256).
+__global__ __launch_bounds__(256) void sum_parts_kernel(const float* __restrict__ part,
+                                                        int nsplit, int64_t n,
+                                                        float* __restrict__ dw) {
+    __shared__ float red[4][64];
+    const int il = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + il;
+    float s = 0.0f;
+    if (i < n)
+        for (int k = g; k < nsplit; k += 4) s += part[(int64_t)k * n + i];
+    red[g][il] = s;
+    __syncthreads();
+    if (g == 0 && i < n) dw[i] = ((red[0][il] + red[1][il]) + red[2][il]) + red[3][il];
 }
+static unsigned sum_parts_grid(int64_t n) { return (unsigned)((n + 63) / 64); }
 
 // Partial bias gradients: part[b][co] = sum of dy over the b-th of gridDim.y
 // strided shares of channel co's plane (channel-major copy; padding is zero),
@@ -824,6 +833,130 @@ __global__ void up_bwd_kernel(const u16* __restrict__ dy, int dcs, int dco, u16*
         }
         if (mask && !(from_bf16(mask[p * mcs + mco + c]) > 0.0f)) s = 0.0f;
         dx[p * xcs + xco + c] = to_bf16(s);
+    }
+}
+
+// The same four operations on 8 channels (16 bytes) per thread with 32-bit
+// index arithmetic, each channel's arithmetic in the same order as above (the
+// same bits): used when the channel count, strides and offsets are multiples
+// of 8, the pointers 16-byte aligned and the element count below 2^31.
+__device__ __forceinline__ void unpack8(uint4 v, float f[8]) {
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        f[2 * k] = __uint_as_float(w[k] << 16);
+        f[2 * k + 1] = __uint_as_float(w[k] & 0xFFFF0000u);
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8]) {
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        w[k] = (uint32_t)to_bf16(f[2 * k]) | ((uint32_t)to_bf16(f[2 * k + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+__device__ __forceinline__ const uint4& at8(const u16* p, int64_t e) {
+    return *reinterpret_cast<const uint4*>(p + e);
+}
+
+__global__ __launch_bounds__(256) void pool8_kernel(const u16* __restrict__ x, int xcs, int xco,
+                                                    u16* __restrict__ y, int ycs, int yco,
+                                                    int C8, int nz, int ny, int nx) {
+    const int hy = ny / 2, hx = nx / 2;
+    const int n = (nz / 2) * hy * hx * C8;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c8 = i % C8, p = i / C8;
+        const int px = p % hx, t = p / hx, py = t % hy, pz = t / hy;
+        float s[8], f[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int v = ((2 * pz + (q >> 2)) * ny + 2 * py + ((q >> 1) & 1)) * nx + 2 * px +
+                          (q & 1);
+            unpack8(at8(x, (int64_t)v * xcs + xco + 8 * c8), f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s[k] += f[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] *= 0.125f;
+        *reinterpret_cast<uint4*>(y + (int64_t)p * ycs + yco + 8 * c8) = pack8(s);
+    }
+}
+
+__global__ __launch_bounds__(256) void pool_bwd8_kernel(const u16* __restrict__ dy, int dcs,
+                                                        int dco, u16* __restrict__ dxp, int xcs,
+                                                        int xco, const u16* __restrict__ mask,
+                                                        int mcs, int mco, int accumulate, int C8,
+                                                        int nz, int ny, int nx) {
+    const int n = nz * ny * nx * C8;
+    const int hy = ny / 2, hx = nx / 2;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c8 = i % C8, v = i / C8;
+        const int xx = v % nx, t = v / nx, yy = t % ny, zz = t / ny;
+        const int p = ((zz / 2) * hy + yy / 2) * hx + xx / 2;
+        float g[8], m[8], d[8];
+        unpack8(at8(dy, (int64_t)p * dcs + dco + 8 * c8), g);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) g[k] *= 0.125f;
+        if (mask) {
+            unpack8(at8(mask, (int64_t)v * mcs + mco + 8 * c8), m);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (!(m[k] > 0.0f)) g[k] = 0.0f;
+        }
+        u16* o = dxp + (int64_t)v * xcs + xco + 8 * c8;
+        if (accumulate) {
+            unpack8(*reinterpret_cast<const uint4*>(o), d);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) g[k] = d[k] + g[k];
+        }
+        *reinterpret_cast<uint4*>(o) = pack8(g);
+    }
+}
+
+__global__ __launch_bounds__(256) void up8_kernel(const u16* __restrict__ x, int xcs, int xco,
+                                                  u16* __restrict__ y, int ycs, int yco, int C8,
+                                                  int nz, int ny, int nx) {
+    const int n = nz * ny * nx * C8;
+    const int hy = ny / 2, hx = nx / 2;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c8 = i % C8, v = i / C8;
+        const int xx = v % nx, t = v / nx, yy = t % ny, zz = t / ny;
+        const int p = ((zz / 2) * hy + yy / 2) * hx + xx / 2;
+        *reinterpret_cast<uint4*>(y + (int64_t)v * ycs + yco + 8 * c8) =
+            at8(x, (int64_t)p * xcs + xco + 8 * c8);
+    }
+}
+
+__global__ __launch_bounds__(256) void up_bwd8_kernel(const u16* __restrict__ dy, int dcs,
+                                                      int dco, u16* __restrict__ dx, int xcs,
+                                                      int xco, const u16* __restrict__ mask,
+                                                      int mcs, int mco, int C8, int nz, int ny,
+                                                      int nx) {
+    const int hy = ny / 2, hx = nx / 2;
+    const int n = (nz / 2) * hy * hx * C8;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c8 = i % C8, p = i / C8;
+        const int px = p % hx, t = p / hx, py = t % hy, pz = t / hy;
+        float s[8], f[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int v = ((2 * pz + (q >> 2)) * ny + 2 * py + ((q >> 1) & 1)) * nx + 2 * px +
+                          (q & 1);
+            unpack8(at8(dy, (int64_t)v * dcs + dco + 8 * c8), f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s[k] += f[k];
+        }
+        if (mask) {
+            unpack8(at8(mask, (int64_t)p * mcs + mco + 8 * c8), f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (!(f[k] > 0.0f)) s[k] = 0.0f;
+        }
+        *reinterpret_cast<uint4*>(dx + (int64_t)p * xcs + xco + 8 * c8) = pack8(s);
     }
 }
 
@@ -1089,11 +1222,11 @@ static hipError_t wgrad_lds_t(const void* x, int xcs, int xco, const void* dy, i
     hipLaunchKernelGGL(kern, dim3((unsigned)g.groups), dim3(256), lds, st, (const u16*)x, xcs,
                        xco, (const u16*)dy, dcs, dco, part, bpart, nz, ny, nx, g.tiles_x,
                        g.tiles_y, g.zc);
-    hipLaunchKernelGGL(sum_parts_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const float*)part,
-                       g.groups, n, dw);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(sum_parts_grid(n)), dim3(256), 0, st,
+                       (const float*)part, g.groups, n, dw);
     if (db)
-        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, st, (const float*)bpart,
-                           g.groups, (int64_t)16, db);
+        hipLaunchKernelGGL(sum_parts_kernel, dim3(sum_parts_grid(16)), dim3(256), 0, st,
+                           (const float*)bpart, g.groups, (int64_t)16, db);
     return hipGetLastError();
 }
 
@@ -1141,21 +1274,39 @@ hipError_t launch_cnn_wgrad(int cin, int cout, const void* x, int xcs, int xco, 
     return hipErrorInvalidValue;
     if (e != hipSuccess) return e;
     const int64_t n = (int64_t)cout * 27 * cin;
-    hipLaunchKernelGGL(sum_parts_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const float*)part,
-                       nsplit, n, dw);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(sum_parts_grid(n)), dim3(256), 0, st,
+                       (const float*)part, nsplit, n, dw);
     if (db) {
         float* bpart = part + (int64_t)nsplit * n;
         hipLaunchKernelGGL(bias_grad_kernel, dim3(cout, BIAS_SPLIT), dim3(256), 0, st,
                            (const u16*)dy_cm, plane, bpart);
-        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, st, (const float*)bpart,
-                           BIAS_SPLIT, (int64_t)cout, db);
+        hipLaunchKernelGGL(sum_parts_kernel, dim3(sum_parts_grid(cout)), dim3(256), 0, st,
+                           (const float*)bpart, BIAS_SPLIT, (int64_t)cout, db);
     }
     return hipGetLastError();
+}
+
+// 16-byte access allowed: channel count, strides, offsets and pointers, and
+// the 32-bit index arithmetic of the vector kernels (nvox (C / 8) < 2^31 for
+// the nvox voxels of the finer level)
+static bool vec8(int C, int64_t nvox, std::initializer_list<int> cs_co,
+                 std::initializer_list<const void*> ptrs) {
+    if (C % 8 || nvox * (C / 8) >= ((int64_t)1 << 31)) return false;
+    for (int v : cs_co)
+        if (v % 8) return false;
+    for (const void* p : ptrs)
+        if ((uintptr_t)p % 16) return false;
+    return true;
 }
 
 hipError_t launch_cnn_pool(const void* x, int xcs, int xco, void* y, int ycs, int yco, int C,
                            int nz, int ny, int nx, hipStream_t st) {
     const int64_t n = (int64_t)(nz / 2) * (ny / 2) * (nx / 2) * C;
+    if (vec8(C, (int64_t)nz * ny * nx, {xcs, xco, ycs, yco}, {x, y})) {
+        hipLaunchKernelGGL(pool8_kernel, dim3(grid_for(n / 8)), dim3(256), 0, st,
+                           (const u16*)x, xcs, xco, (u16*)y, ycs, yco, C / 8, nz, ny, nx);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(pool_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const u16*)x, xcs, xco,
                        (u16*)y, ycs, yco, C, nz, ny, nx);
     return hipGetLastError();
@@ -1165,6 +1316,13 @@ hipError_t launch_cnn_pool_bwd(const void* dy, int dcs, int dco, void* dx, int x
                                const void* mask, int mcs, int mco, int accumulate, int C, int nz,
                                int ny, int nx, hipStream_t st) {
     const int64_t n = (int64_t)nz * ny * nx * C;
+    if (vec8(C, (int64_t)nz * ny * nx, {dcs, dco, xcs, xco, mask ? mcs : 0, mask ? mco : 0},
+             {dy, dx, mask})) {
+        hipLaunchKernelGGL(pool_bwd8_kernel, dim3(grid_for(n / 8)), dim3(256), 0, st,
+                           (const u16*)dy, dcs, dco, (u16*)dx, xcs, xco, (const u16*)mask, mcs,
+                           mco, accumulate, C / 8, nz, ny, nx);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(pool_bwd_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const u16*)dy, dcs,
                        dco, (u16*)dx, xcs, xco, (const u16*)mask, mcs, mco, accumulate, C, nz, ny,
                        nx);
@@ -1174,6 +1332,11 @@ hipError_t launch_cnn_pool_bwd(const void* dy, int dcs, int dco, void* dx, int x
 hipError_t launch_cnn_up(const void* x, int xcs, int xco, void* y, int ycs, int yco, int C,
                          int nz, int ny, int nx, hipStream_t st) {
     const int64_t n = (int64_t)nz * ny * nx * C;
+    if (vec8(C, (int64_t)nz * ny * nx, {xcs, xco, ycs, yco}, {x, y})) {
+        hipLaunchKernelGGL(up8_kernel, dim3(grid_for(n / 8)), dim3(256), 0, st, (const u16*)x,
+                           xcs, xco, (u16*)y, ycs, yco, C / 8, nz, ny, nx);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(up_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const u16*)x, xcs, xco,
                        (u16*)y, ycs, yco, C, nz, ny, nx);
     return hipGetLastError();
@@ -1183,6 +1346,13 @@ hipError_t launch_cnn_up_bwd(const void* dy, int dcs, int dco, void* dx, int xcs
                              const void* mask, int mcs, int mco, int C, int nz, int ny, int nx,
                              hipStream_t st) {
     const int64_t n = (int64_t)(nz / 2) * (ny / 2) * (nx / 2) * C;
+    if (vec8(C, (int64_t)nz * ny * nx, {dcs, dco, xcs, xco, mask ? mcs : 0, mask ? mco : 0},
+             {dy, dx, mask})) {
+        hipLaunchKernelGGL(up_bwd8_kernel, dim3(grid_for(n / 8)), dim3(256), 0, st,
+                           (const u16*)dy, dcs, dco, (u16*)dx, xcs, xco, (const u16*)mask, mcs,
+                           mco, C / 8, nz, ny, nx);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(up_bwd_kernel, dim3(grid_for(n)), dim3(256), 0, st, (const u16*)dy, dcs,
                        dco, (u16*)dx, xcs, xco, (const u16*)mask, mcs, mco, C, nz, ny, nx);
     return hipGetLastError();

@@ -241,6 +241,76 @@ def test_pool_and_upsample_exact(dev):
     np.testing.assert_array_equal(from_dev_bf16(u).reshape(4, 6, 8, 32)[..., 16:], C.up(got))
 
 
+def test_pool_up_and_gradients_vector_and_scalar_paths(dev):
+    """Pooling, upsampling and their gradients (mask and accumulate) run on
+    8 channels per thread when channel offsets/strides are multiples of 8 and
+    one channel per thread otherwise (offset 4 here): the same bits, and the
+    float32 restatement (children summed in order, 1/8, ReLU mask, the
+    accumulated sum rounded to BF16)."""
+    net = W.WarmStartNet(dev)
+    L = net._L
+    rng = np.random.default_rng(12)
+    nz, ny, nx, Cc = 4, 6, 10, 16
+    fine = (nz, ny, nx, Cc)
+    coarse = (nz // 2, ny // 2, nx // 2, Cc)
+
+    def dev_view(a, cs, co):
+        buf = np.zeros(a.shape[:3] + (cs,))
+        buf[..., co:co + Cc] = a
+        return to_dev_bf16(buf.reshape(-1), dev)
+
+    def view(t, shape, cs, co):
+        return from_dev_bf16(t).reshape(shape[:3] + (cs,))[..., co:co + Cc]
+
+    xf = bf16_round(rng.standard_normal(fine))
+    xc = bf16_round(rng.standard_normal(coarse))
+    mf = bf16_round(rng.standard_normal(fine))
+    mc = bf16_round(rng.standard_normal(coarse))
+    acc0 = bf16_round(rng.standard_normal(fine))
+    f32 = np.float32
+    # float32 restatements
+    ch = [(a, b, c) for a in (0, 1) for b in (0, 1) for c in (0, 1)]
+    s = np.zeros(coarse, f32)
+    for a, b, c in ch:
+        s = s + xf[a::2, b::2, c::2].astype(f32)
+    want_pool = bf16_round(s * f32(0.125))
+    want_up = np.repeat(np.repeat(np.repeat(xc, 2, 0), 2, 1), 2, 2)
+    g = np.repeat(np.repeat(np.repeat(xc.astype(f32) * f32(0.125), 2, 0), 2, 1), 2, 2)
+    g = np.where(mf > 0, g, f32(0))
+    want_pool_bwd = bf16_round(acc0.astype(f32) + g)
+    s = np.zeros(coarse, f32)
+    for a, b, c in ch:
+        s = s + xf[a::2, b::2, c::2].astype(f32)
+    want_up_bwd = bf16_round(np.where(mc > 0, s, f32(0)))
+    results = {}
+    for cs, co in ((16, 0), (24, 4)):  # vector path, scalar path
+        out = {}
+        y = torch.zeros(int(np.prod(coarse[:3])) * cs, dtype=torch.int16, device=dev)
+        net._check(L.ostr_cnn_pool2(W._ptr(dev_view(xf, cs, co)), cs, co, W._ptr(y), cs, co, Cc,
+                                    nz, ny, nx, None))
+        out["pool"] = view(y, coarse, cs, co)
+        u = torch.zeros(int(np.prod(fine[:3])) * cs, dtype=torch.int16, device=dev)
+        net._check(L.ostr_cnn_up2(W._ptr(dev_view(xc, cs, co)), cs, co, W._ptr(u), cs, co, Cc,
+                                  nz, ny, nx, None))
+        out["up"] = view(u, fine, cs, co)
+        dx = dev_view(acc0, cs, co)
+        mfd, dyc = dev_view(mf, cs, co), dev_view(xc, cs, co)
+        net._check(L.ostr_cnn_pool2_bwd(W._ptr(dyc), cs, co, W._ptr(dx), cs, co, W._ptr(mfd), cs,
+                                        co, 1, Cc, nz, ny, nx, None))
+        out["pool_bwd"] = view(dx, fine, cs, co)
+        dxc = torch.zeros(int(np.prod(coarse[:3])) * cs, dtype=torch.int16, device=dev)
+        mcd, dyf = dev_view(mc, cs, co), dev_view(xf, cs, co)
+        net._check(L.ostr_cnn_up2_bwd(W._ptr(dyf), cs, co, W._ptr(dxc), cs, co, W._ptr(mcd), cs,
+                                      co, Cc, nz, ny, nx, None))
+        out["up_bwd"] = view(dxc, coarse, cs, co)
+        torch.cuda.synchronize()
+        results[co] = out
+    for k, want in (("pool", want_pool), ("up", want_up), ("pool_bwd", want_pool_bwd),
+                    ("up_bwd", want_up_bwd)):
+        np.testing.assert_array_equal(results[0][k], results[4][k], err_msg=k)
+        np.testing.assert_array_equal(results[0][k], want, err_msg=k)
+
+
 def test_training_reduces_loss(dev):
     """Adam steps on one sample with a learnable target (an affine map of the
     input) lower the loss several-fold (smoke test of the training loop)."""
