@@ -46,17 +46,19 @@ int ostr_cnn_conv3d(int32_t cin, int32_t cout, const void* x, int32_t xcs, int32
                     const void* mask, int32_t mcs, int32_t mco, int32_t relu, int32_t nz,
                     int32_t ny, int32_t nx, void* stream);
 
-/* Workspace (bytes) of ostr_cnn_wgrad for these sizes and nsplit partials. */
+/* Workspace (bytes) of ostr_cnn_wgrad for these sizes (one partial sum per
+ * work-group of a fixed grid); nsplit is accepted and not used. 0 for an
+ * unsupported shape. */
 size_t ostr_cnn_wgrad_workspace(int32_t cin, int32_t cout, int32_t nz, int32_t ny, int32_t nx,
                                 int32_t nsplit);
 
 /* dw [cout][27][cin] = gradient of the convolution weights for input x and
  * output gradient dy (FP32, written); db [cout] likewise (may be NULL).
- * Deterministic: partial sums added in a fixed order. The shapes (8,16),
- * (16,16) and (48,16) run from LDS (one partial per work-group of a fixed
- * grid; nsplit unused) and need channel strides/offsets that are multiples of
- * 8 and 16-byte aligned x and dy (OSTR_E_ARG otherwise); the others reduce
- * nsplit (1..1024) row shares from channel-major copies in the workspace. */
+ * Deterministic: one partial sum per work-group of a fixed grid, added in a
+ * fixed order (nsplit, 1..1024, is accepted and not used). Channel strides and
+ * offsets multiples of 8, x and dy 16-byte aligned (OSTR_E_ARG otherwise).
+ * (cin, cout) one of (8,16) (16,16) (48,16) (16,32) (32,32) (32,64) (64,64)
+ * (96,32). */
 int ostr_cnn_wgrad(int32_t cin, int32_t cout, const void* x, int32_t xcs, int32_t xco,
                    const void* dy, int32_t dcs, int32_t dco, int32_t nz, int32_t ny, int32_t nx,
                    float* dw, float* db, void* work, size_t work_bytes, int32_t nsplit,

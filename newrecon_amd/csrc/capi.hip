@@ -1517,7 +1517,9 @@ OSTR_API int ostr_cnn_conv3d(int32_t cin, int32_t cout, const void* x, int32_t x
 
 OSTR_API size_t ostr_cnn_wgrad_workspace(int32_t cin, int32_t cout, int32_t nz, int32_t ny,
                                          int32_t nx, int32_t nsplit) {
-    return ostr::cnn_wgrad_work_bytes(cin, cout, nz, ny, nx, nsplit);
+    (void)nsplit;  // kept for the interface; the partials follow a fixed grid
+    if (!ostr::cnn_wgrad_supported(cin, cout) || !dims_ok(nz, ny, nx)) return 0;
+    return ostr::cnn_wgrad_work_bytes(cin, cout, nz, ny, nx);
 }
 
 OSTR_API int ostr_cnn_wgrad(int32_t cin, int32_t cout, const void* x, int32_t xcs, int32_t xco,
@@ -1531,15 +1533,13 @@ OSTR_API int ostr_cnn_wgrad(int32_t cin, int32_t cout, const void* x, int32_t xc
         return cnn_fail(OSTR_E_ARG, "ostr_cnn_wgrad: bad argument");
     if (xco + cin > xcs || dco + cout > dcs)
         return cnn_fail(OSTR_E_ARG, "ostr_cnn_wgrad: channel view out of range");
-    if (ostr::cnn_wgrad_lds_shape(cin, cout) &&
-        (xcs % 8 || xco % 8 || dcs % 8 || dco % 8 || (uintptr_t)x % 16 || (uintptr_t)dy % 16))
-        return cnn_fail(OSTR_E_ARG, "ostr_cnn_wgrad: %dx%d needs channel strides/offsets that "
-                                    "are multiples of 8 and 16-byte aligned x and dy",
-                        cin, cout);
-    if (work_bytes < ostr::cnn_wgrad_work_bytes(cin, cout, nz, ny, nx, nsplit))
+    if (xcs % 8 || xco % 8 || dcs % 8 || dco % 8 || (uintptr_t)x % 16 || (uintptr_t)dy % 16)
+        return cnn_fail(OSTR_E_ARG, "ostr_cnn_wgrad: channel strides/offsets must be multiples "
+                                    "of 8 and x, dy 16-byte aligned");
+    if (work_bytes < ostr::cnn_wgrad_work_bytes(cin, cout, nz, ny, nx))
         return cnn_fail(OSTR_E_RANGE, "ostr_cnn_wgrad: workspace too small");
     return cnn_done(ostr::launch_cnn_wgrad(cin, cout, x, xcs, xco, dy, dcs, dco, nz, ny, nx, dw,
-                                           db, work, nsplit, (hipStream_t)stream),
+                                           db, work, (hipStream_t)stream),
                     "ostr_cnn_wgrad");
 }
 

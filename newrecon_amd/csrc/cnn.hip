@@ -357,104 +357,36 @@ __global__ __launch_bounds__(256 + 64 * conv_loaders(conv_slab(CIN, R, XW))) voi
     }
 }
 
-// ---- weight gradient -----------------------------------------------------
-// Channel-major, row-padded copy [C16][nz][ny][nxp] (C16 = C rounded up to
-// 16, extra channels zero) of a channels-last view, shifted by dx along x
-// (element x holds voxel x + dx, zero outside [0, nx) and for x >= nx).
-__global__ void to_cm_kernel(const u16* __restrict__ src, int cs, int co, int C, int C16,
-                             u16* __restrict__ dst, int nz, int ny, int nx, int nxp, int dx) {
-    const int64_t n = (int64_t)C16 * nz * ny * nxp;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int xx = (int)(i % nxp);
-        int64_t r = i / nxp;
-        const int row = (int)(r % ((int64_t)nz * ny));
-        const int c = (int)(r / ((int64_t)nz * ny));
-        const int xs = xx + dx;
-        u16 val = 0;
-        if (c < C && xx < nx && (unsigned)xs < (unsigned)nx)
-            val = src[((int64_t)row * nx + xs) * cs + co + c];
-        dst[i] = val;
-    }
-}
-
-// The same copies through an LDS tile, NCOPY at once: work-group b converts
-// the 32-voxel segment (row b / (nxp / 32), columns 32 (b % (nxp / 32)) ..)
-// of every channel. The segment and its two neighbouring voxels are read
-// channels-last, 16 bytes (8 channels of one voxel) per load, into tile
-// [C16][34] (tile column p holds voxel x0 - 1 + p, zero outside [0, nx));
-// copy s (shift dx = s - 1 when NCOPY is 3, 0 when it is 1) is written
-// channel-major 16 bytes (8 voxels of one channel) per store. C and co must
-// be multiples of 8 (cs too), C16 <= 96. Same values as to_cm_kernel.
-constexpr int TO_CM_MAX_C = 96;
-template <int NCOPY>
-__global__ __launch_bounds__(256) void to_cm_tile_kernel(const u16* __restrict__ src, int cs,
-                                                         int co, int C, int C16,
-                                                         u16* __restrict__ dst, int nz, int ny,
-                                                         int nx, int nxp) {
-    __shared__ u16 tile[TO_CM_MAX_C][34];
-    const int xchunks = nxp / 32;
-    const int row = blockIdx.x / xchunks, x0 = (blockIdx.x % xchunks) * 32;
-    const int64_t rows = (int64_t)nz * ny;
-    const int cg8 = C / 8;
-    for (int t = threadIdx.x; t < 34 * cg8; t += 256) {
-        const int p = t / cg8, cg = t % cg8;
-        const int x = x0 - 1 + p;
-        uint4 v = {0u, 0u, 0u, 0u};
-        if ((unsigned)x < (unsigned)nx)
-            v = *reinterpret_cast<const uint4*>(src + ((int64_t)row * nx + x) * cs + co + cg * 8);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            tile[cg * 8 + 2 * k][p] = (u16)(w[k] & 0xFFFFu);
-            tile[cg * 8 + 2 * k + 1][p] = (u16)(w[k] >> 16);
-        }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < NCOPY * C16 * 4; t += 256) {
-        const int s = t / (C16 * 4), r = t % (C16 * 4);
-        const int c = r / 4, xx = (r % 4) * 8;
-        const int dxs = NCOPY == 3 ? s - 1 : 0;
-        u16 o[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int xo = x0 + xx + k;
-            o[k] = (c < C && xo < nx) ? tile[c][xx + k + dxs + 1] : (u16)0;
-        }
-        uint4 v;
-        v.x = (uint32_t)o[0] | ((uint32_t)o[1] << 16);
-        v.y = (uint32_t)o[2] | ((uint32_t)o[3] << 16);
-        v.z = (uint32_t)o[4] | ((uint32_t)o[5] << 16);
-        v.w = (uint32_t)o[6] | ((uint32_t)o[7] << 16);
-        *reinterpret_cast<uint4*>(dst + ((int64_t)s * C16 + c) * rows * nxp +
-                                  (int64_t)row * nxp + x0 + xx) = v;
-    }
-}
-
-// ---- weight gradient through an LDS ring (COUT = 16) ----------------------
+// ---- weight gradient through an LDS ring ------------------------------------
 // dW[co][tap][ci] = sum_v dY[v][co] X[v + off(tap)][ci] and db[co] =
 // sum_v dY[v][co] reduce over voxels, so both MFMA operands hold 8
-// consecutive voxels of one channel (K = 32 voxels of one row half). They
-// are read from channels-last LDS slabs with the transposing read
-// ds_read_b64_tr_b16: within each 16 lanes, lane i receives element i & 3 of
-// the 4 that source lanes (i >> 2) + 4 e (e = 0..3) address, so when source
-// lane s addresses channels 4 (s & 3) .. + 3 of voxel v0 + (s >> 2), lane i
-// receives voxels v0 .. v0 + 3 of channel i (two reads: 8 voxels); any voxel
-// shift works, so the 27 taps need no shifted copies. Work-group: a 64 x 8
-// (x, y) tile (64 x 4 for 48 input channels) marching in z over [z0, z0 + zc) with the X slabs of slices
-// z - 1 .. z + 1 (halo included, zero outside the volume) in a 3-slot ring and
-// dY of slice z in a 2-slot ring, both in the convolution's [c8][row][x]
-// layout, the next slices fetched into registers during compute (no global
-// load is waited on inside the step); the 27 NCI (tap,
-// 16-channel block) accumulators are dealt to the 4 compute waves round-robin
-// (wave 0 also sums dY against ones for the bias). Each group writes its
-// partial sums: part[g][co][tap CIN + ci], bpart[g][co]. Channels past CIN
-// (CIN = 8) read channel 0..7 again; their columns are not written.
-// x by y tile: 64 x 8 voxels for up to 16 input channels, 64 x 4 for 48
-constexpr int WG_XW = 64;
-constexpr int wgrad_ty(int cin) { return cin <= 16 ? 8 : 4; }
-constexpr int wgrad_slab(int cin) { return cin / 8 * (wgrad_ty(cin) + 2) * (WG_XW + 2); }
-constexpr int wgrad_dslab(int cin) { return 2 * wgrad_ty(cin) * WG_XW; }
+// consecutive voxels of one channel (K = 32 voxels of one row). They are read
+// from channels-last LDS slabs with the transposing read ds_read_b64_tr_b16:
+// within each 16 lanes, lane i receives element i & 3 of the 4 that source
+// lanes (i >> 2) + 4 e (e = 0..3) address, so when source lane s addresses
+// channels 4 (s & 3) .. + 3 of voxel v0 + (s >> 2), lane i receives voxels
+// v0 .. v0 + 3 of channel i (two reads: 8 voxels); any voxel shift works, so
+// the 27 taps need no shifted copies. Work-group: an XW x TY (x, y) tile
+// marching in z over [z0, z0 + zc) with the X slabs of slices z - 1 .. z + 1
+// (halo included, zero outside the volume) in a 3-slot ring and dY of slice z
+// in a 2-slot ring, both in the convolution's [c8][row][x] layout, the next
+// slices fetched into registers during compute (no global load is waited on
+// inside the step). blockIdx.y picks one 16-channel block of the output
+// (cob) and one share (tg) of its 27 NCI (tap, 16-channel input block)
+// accumulators, which are dealt to the 4 waves round-robin; every wave also
+// sums dY against ones for the bias (share 0's wave 0 keeps it). Each
+// group writes its partial sums: part[g][co][tap CIN + ci], bpart[g][co].
+// Channels past CIN (CIN = 8) read channels 0..7 again; their columns are not
+// written.
+constexpr int wgrad_xw(int cin) { return cin >= 64 ? 32 : 64; }
+constexpr int wgrad_ty(int cin, int cout) { return cin <= 16 && cout <= 16 ? 8 : 4; }
+constexpr int wgrad_slab(int cin, int cout) {
+    return cin / 8 * (wgrad_ty(cin, cout) + 2) * (wgrad_xw(cin) + 2);
+}
+constexpr int wgrad_dslab(int cin, int cout) { return 2 * wgrad_ty(cin, cout) * wgrad_xw(cin); }
+// accumulator tiles of one output block, their shares (at most 28 per wave)
+constexpr int wgrad_tiles(int cin) { return 27 * ((cin + 15) / 16); }
+constexpr int wgrad_shares(int cin) { return (wgrad_tiles(cin) + 111) / 112; }
 
 typedef short v4i16 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ bf16x8 tr_read8(const uint4* base, int off0, int off1) {
@@ -467,19 +399,23 @@ __device__ __forceinline__ bf16x8 tr_read8(const uint4* base, int off0, int off1
     return __builtin_bit_cast(bf16x8, both);
 }
 
-template <int CIN>
+template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void wgrad_lds_kernel(
     const u16* __restrict__ x, int xcs, int xco, const u16* __restrict__ dy, int dcs, int dco,
     float* __restrict__ part, float* __restrict__ bpart, int nz, int ny, int nx, int tiles_x,
     int tiles_y, int zc) {
-    constexpr int COUT = 16, NC8 = CIN / 8, NCI = (CIN + 15) / 16, NTILE = 27 * NCI;
-    constexpr int WG_TY = wgrad_ty(CIN), TR = WG_TY + 2, TW = WG_XW + 2;
-    constexpr int SLAB = wgrad_slab(CIN), DSLAB = wgrad_dslab(CIN);
-    constexpr int NTW = (NTILE + 3) / 4;  // accumulator tiles per wave
-    extern __shared__ uint4 lds[];      // 3 X slabs, then 2 dY slabs
+    constexpr int NC8 = CIN / 8, NCI = (CIN + 15) / 16;
+    constexpr int XW = wgrad_xw(CIN), WG_TY = wgrad_ty(CIN, COUT), KPR = XW / 32;
+    constexpr int TR = WG_TY + 2, TW = XW + 2;
+    constexpr int SLAB = wgrad_slab(CIN, COUT), DSLAB = wgrad_dslab(CIN, COUT);
+    constexpr int TC = wgrad_tiles(CIN), TG = wgrad_shares(CIN);
+    constexpr int TPW = (TC + TG - 1) / TG;  // tiles per work-group
+    constexpr int NTW = (TPW + 3) / 4;       // tiles per wave
+    extern __shared__ uint4 lds[];           // 3 X slabs, then 2 dY slabs
     uint4* xs = lds;
     uint4* ds = lds + 3 * SLAB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cob = (int)blockIdx.y / TG, tg = (int)blockIdx.y % TG;
     const int n_groups = (int)gridDim.x;
     int t;
     {
@@ -490,14 +426,15 @@ __global__ __launch_bounds__(256) void wgrad_lds_kernel(
     const int tx = t % tiles_x;
     t /= tiles_x;
     const int ty = t % tiles_y, tz = t / tiles_y;
-    const int x0 = tx * WG_XW, y0 = ty * WG_TY, z0 = tz * zc;
+    const int x0 = tx * XW, y0 = ty * WG_TY, z0 = tz * zc;
     const int z1 = z0 + zc < nz ? z0 + zc : nz;
+    const int dco_b = dco + 16 * cob;
 
     // This thread's slab elements e = threadIdx.x + 256 i: X slab elements
     // (e < SLAB, [c8][row][x] with the halo) and dY slab elements (SLAB <= e <
-    // SLAB + DSLAB); their offsets inside a slice (-1: outside the volume's x/y
-    // range, zero) and their LDS index inside a slot, computed once. The next
-    // slabs are fetched into registers while the current slice is computed.
+    // SLAB + DSLAB, this group's 16 output channels); their offsets inside a
+    // slice (-1: outside the volume's x/y range, zero) and their LDS index
+    // inside a slot, computed once.
     constexpr int NE = (SLAB + DSLAB + 255) / 256;
     int soff[NE], loff[NE];
 #pragma unroll
@@ -515,10 +452,10 @@ __global__ __launch_bounds__(256) void wgrad_lds_kernel(
         } else if (e < SLAB + DSLAB) {
             const int f = e - SLAB;
             const int c8 = f & 1, rest = f >> 1;
-            const int xi = rest % WG_XW, row = rest / WG_XW;
+            const int xi = rest % XW, row = rest / XW;
             const int gx = x0 + xi, gy = y0 + row;
-            if (gx < nx && gy < ny) soff[i] = (gy * nx + gx) * dcs + dco + c8 * 8;
-            loff[i] = (c8 * WG_TY + row) * WG_XW + xi;
+            if (gx < nx && gy < ny) soff[i] = (gy * nx + gx) * dcs + dco_b + c8 * 8;
+            loff[i] = (c8 * WG_TY + row) * XW + xi;
         }
     }
     const int64_t xslice = (int64_t)ny * nx * xcs, dslice = (int64_t)ny * nx * dcs;
@@ -572,18 +509,19 @@ __global__ __launch_bounds__(256) void wgrad_lds_kernel(
         if (more) fetch(z + 2, z + 1);
         const uint4* dslot = ds + ((z + 2) % 2) * DSLAB;
 #pragma unroll 1
-        for (int kc = 0; kc < 2 * WG_TY; ++kc) {
-            const int r = kc >> 1, xh = (kc & 1) * 32;
-            // A: dY of voxels (x0 + xh + 8 kg .. + 7, row r), channels 0..15
-            const int aoff = ((pc * WG_TY + r) * WG_XW + xh + vx) * 16 + hb;
+        for (int kc = 0; kc < KPR * WG_TY; ++kc) {
+            const int r = kc / KPR, xh = (kc % KPR) * 32;
+            // A: dY of voxels (x0 + xh + 8 kg .. + 7, row r), channels 16 cob .. + 15
+            const int aoff = ((pc * WG_TY + r) * XW + xh + vx) * 16 + hb;
             const bf16x8 a = tr_read8(dslot, aoff, aoff + 4 * 16);
             // branch-free, so that the reads of all tiles can be issued ahead:
-            // every wave sums the bias (wave 0's is kept), and a wave's slots
-            // past the last tile repeat it (not written)
+            // every wave sums the bias, and a wave's slots past its share's
+            // last tile repeat a tile (not written)
             accb = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ones, accb, 0, 0, 0);
 #pragma unroll
             for (int i = 0; i < NTW; ++i) {
-                const int u = wave + 4 * i < NTILE ? wave + 4 * i : NTILE - 1;
+                const int ul = wave + 4 * i < TPW ? wave + 4 * i : TPW - 1;
+                const int u = tg * TPW + ul < TC ? tg * TPW + ul : TC - 1;
                 const int tap = u / NCI, cb = u % NCI;
                 const int dzi = tap / 9, dyi = (tap / 3) % 3, dxi = tap % 3;
                 const int c8 = 2 * cb + pc < NC8 ? 2 * cb + pc : NC8 - 1;
@@ -599,127 +537,23 @@ __global__ __launch_bounds__(256) void wgrad_lds_kernel(
             __syncthreads();
         }
     }
-    // D row co = 4 kg + e, column ci = lane & 15
+    // D row co = 16 cob + 4 kg + e, column ci = lane & 15
     constexpr int K = 27 * CIN;
     float* out = part + (int64_t)group * COUT * K;
 #pragma unroll
     for (int i = 0; i < NTW; ++i) {
-        const int u = wave + 4 * i;
-        if (u >= NTILE) break;
+        const int ul = wave + 4 * i, u = tg * TPW + ul;
+        if (ul >= TPW || u >= TC) continue;
         const int tap = u / NCI, ci = (u % NCI) * 16 + (lane & 15);
         if (ci < CIN) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) out[(int64_t)(4 * kg + e) * K + tap * CIN + ci] = acc[i][e];
+            for (int e = 0; e < 4; ++e)
+                out[(int64_t)(16 * cob + 4 * kg + e) * K + tap * CIN + ci] = acc[i][e];
         }
     }
-    if (wave == 0 && (lane & 15) == 0) {
+    if (tg == 0 && wave == 0 && (lane & 15) == 0) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) bpart[(int64_t)group * COUT + 4 * kg + e] = accb[e];
-    }
-}
-
-// Partial weight gradient over the rows of split blockIdx.x: part[split][co]
-// [tap * CIN + ci]. dy_cm: [COUT16][rows][nxp]; xs_cm: 3 shifted copies
-// [3][CIN16][rows][nxp] (copy s holds dx = s - 1). TX = 1: one tap per
-// work-group (blockIdx.y = tap); TX = 3: the three taps dx = -1, 0, 1 of one
-// (dz, dy) (blockIdx.y = (dz + 1) * 3 + dy + 1) share each dY operand. A wave
-// takes the 32-voxel chunks q = wave, wave + 4, ... of its rows in order and
-// loads chunk q + 4's operands before the MFMAs of chunk q (double buffer);
-// chunks whose shifted row falls outside the volume are skipped (their X is
-// zero). The 4 waves' sums are combined in LDS in a fixed order.
-template <int CIN, int COUT, int TX>
-__global__ __launch_bounds__(256) void wgrad_kernel(const u16* __restrict__ dy_cm,
-                                                    const u16* __restrict__ xs_cm,
-                                                    float* __restrict__ part, int nz, int ny,
-                                                    int nxp, int rows_per_split) {
-    constexpr int CI16 = (CIN + 15) / 16 * 16, NCI = CI16 / 16, NCO = COUT / 16;
-    constexpr int K = 27 * CIN;
-    constexpr int NACC = TX * NCO * NCI;
-    __shared__ float red[4][NACC * 4][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int j = lane & 15, kg = lane >> 4;
-    const int split = blockIdx.x;
-    const int dz = TX == 1 ? (int)blockIdx.y / 9 - 1 : (int)blockIdx.y / 3 - 1;
-    const int dy = TX == 1 ? ((int)blockIdx.y / 3) % 3 - 1 : (int)blockIdx.y % 3 - 1;
-    const int dx1 = TX == 1 ? (int)blockIdx.y % 3 - 1 : 0;  // the single tap's dx
-    const int64_t rows = (int64_t)nz * ny, plane = rows * nxp;
-    f32x4 acc[TX][NCO][NCI];
-#pragma unroll
-    for (int t = 0; t < TX; ++t)
-#pragma unroll
-        for (int a = 0; a < NCO; ++a)
-#pragma unroll
-            for (int c = 0; c < NCI; ++c) acc[t][a][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const int r0 = split * rows_per_split;
-    const int r1 = (int)((int64_t)r0 + rows_per_split < rows ? r0 + rows_per_split : rows);
-    const int xchunks = nxp / 32;
-    const int nq = r1 > r0 ? (r1 - r0) * xchunks : 0;
-    struct Ops {
-        bf16x8 a[NCO], b[TX][NCI];
-        bool ok;
-    };
-    auto load = [&](int q, Ops& o) {
-        const int row = r0 + q / xchunks, x0 = (q % xchunks) * 32;
-        const int iz = row / ny, iy = row % ny;
-        const int zz = iz + dz, yy = iy + dy;
-        o.ok = (unsigned)zz < (unsigned)nz && (unsigned)yy < (unsigned)ny;
-        if (!o.ok) return;
-        const int64_t rs = (int64_t)zz * ny + yy;
-#pragma unroll
-        for (int a = 0; a < NCO; ++a)
-            o.a[a] = load8(dy_cm + ((int64_t)(a * 16 + j) * rows + row) * nxp + x0 + kg * 8);
-#pragma unroll
-        for (int t = 0; t < TX; ++t) {
-            const u16* xs = xs_cm + (int64_t)((TX == 1 ? dx1 : t - 1) + 1) * CI16 * plane;
-#pragma unroll
-            for (int c = 0; c < NCI; ++c)
-                o.b[t][c] = load8(xs + ((int64_t)(c * 16 + j) * rows + rs) * nxp + x0 + kg * 8);
-        }
-    };
-    auto mma = [&](const Ops& o) {
-        if (!o.ok) return;
-#pragma unroll
-        for (int t = 0; t < TX; ++t)
-#pragma unroll
-            for (int a = 0; a < NCO; ++a)
-#pragma unroll
-                for (int c = 0; c < NCI; ++c)
-                    acc[t][a][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.a[a], o.b[t][c],
-                                                                          acc[t][a][c], 0, 0, 0);
-    };
-    Ops o0, o1;
-    int q = wave;
-    if (q < nq) load(q, o0);
-    while (q < nq) {
-        if (q + 4 < nq) load(q + 4, o1);
-        mma(o0);
-        q += 4;
-        if (q >= nq) break;
-        if (q + 4 < nq) load(q + 4, o0);
-        mma(o1);
-        q += 4;
-    }
-    // combine the 4 waves in a fixed order
-#pragma unroll
-    for (int t = 0; t < TX; ++t)
-#pragma unroll
-        for (int a = 0; a < NCO; ++a)
-#pragma unroll
-            for (int c = 0; c < NCI; ++c)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    red[wave][((t * NCO + a) * NCI + c) * 4 + r][lane] = acc[t][a][c][r];
-    __syncthreads();
-    float* out = part + (int64_t)split * COUT * K;
-    for (int e = threadIdx.x; e < NACC * 4 * 64; e += 256) {
-        const int slot = e / 64, l = e % 64;
-        const float s = ((red[0][slot][l] + red[1][slot][l]) + red[2][slot][l]) + red[3][slot][l];
-        const int t = slot / (NCO * NCI * 4), a = (slot / (NCI * 4)) % NCO, c = (slot / 4) % NCI;
-        const int r = slot % 4;
-        const int tap = TX == 1 ? (int)blockIdx.y : (int)blockIdx.y * 3 + t;
-        const int co = a * 16 + (l >> 4) * 4 + r;  // D row
-        const int ci = c * 16 + (l & 15);          // D column
-        if (ci < CIN) out[(int64_t)co * K + tap * CIN + ci] = s;
+        for (int e = 0; e < 4; ++e) bpart[(int64_t)group * COUT + 16 * cob + 4 * kg + e] = accb[e];
     }
 }
 
@@ -740,22 +574,6 @@ __global__ __launch_bounds__(256) void sum_parts_kernel(const float* __restrict_
     if (g == 0 && i < n) dw[i] = ((red[0][il] + red[1][il]) + red[2][il]) + red[3][il];
 }
 static unsigned sum_parts_grid(int64_t n) { return (unsigned)((n + 63) / 64); }
-
-// Partial bias gradients: part[b][co] = sum of dy over the b-th of gridDim.y
-// strided shares of channel co's plane (channel-major copy; padding is zero),
-// fixed-order reduction; sum_parts_kernel then adds the shares in order.
-constexpr int BIAS_SPLIT = 64;
-__global__ __launch_bounds__(256) void bias_grad_kernel(const u16* __restrict__ dy_cm,
-                                                        int64_t per_channel,
-                                                        float* __restrict__ part) {
-    const u16* p = dy_cm + (int64_t)blockIdx.x * per_channel;
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.y * 256 + threadIdx.x; i < per_channel;
-         i += (int64_t)BIAS_SPLIT * 256)
-        s += from_bf16(p[i]);
-    s = block_sum_256(s);
-    if (threadIdx.x == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = (float)s;
-}
 
 // ---- pooling / upsampling (channels-last views) ---------------------------
 // y (nz/2, ny/2, nx/2) = mean of the 8 children of x (nz, ny, nx).
@@ -1117,20 +935,6 @@ hipError_t conv_t(const u16* x, int xcs, int xco, const u16* w, const float* bia
     return hipGetLastError();
 }
 
-template <int CIN, int COUT>
-hipError_t wgrad_t(const u16* dy_cm, const u16* xs_cm, float* part, int nsplit, int nz, int ny,
-                   int nxp, hipStream_t st) {
-    const int rows = nz * ny;
-    const int per = (rows + nsplit - 1) / nsplit;
-    // the three x-taps per work-group while their accumulators and double
-    // buffered operands stay small
-    constexpr int NMMA = (COUT / 16) * ((CIN + 15) / 16);
-    constexpr int TX = NMMA <= 4 ? 3 : 1;
-    hipLaunchKernelGGL((wgrad_kernel<CIN, COUT, TX>), dim3(nsplit, 27 / TX), dim3(256), 0, st,
-                       dy_cm, xs_cm, part, nz, ny, nxp, per);
-    return hipGetLastError();
-}
-
 }  // namespace
 
 // Supported (CIN, COUT) pairs of the forward and data-gradient convolutions.
@@ -1157,32 +961,25 @@ bool cnn_conv_supported(int cin, int cout) {
     return false;
 }
 
-// Supported (CIN, COUT) pairs of the weight gradient (forward shapes).
+// The weight gradient (wgrad_lds_kernel): supported (CIN, COUT) pairs (the
+// forward shapes), grid and workspace.
 #define OSTR_CNN_WGRAD_SHAPES(X) \
-    X(8, 16) X(16, 16) X(16, 32) X(32, 32) X(32, 64) X(64, 64) X(96, 32) X(48, 16)
-
+    X(8, 16) X(16, 16) X(48, 16) X(16, 32) X(32, 32) X(32, 64) X(64, 64) X(96, 32)
 bool cnn_wgrad_supported(int cin, int cout) {
 #define CASE(A, B) if (cin == A && cout == B) return true;
     OSTR_CNN_WGRAD_SHAPES(CASE)
 #undef CASE
     return false;
 }
-
-int cnn_nxp(int nx) { return (nx + 31) / 32 * 32; }
-
-// The LDS-ring weight gradient (wgrad_lds_kernel): shapes, grid and workspace.
-bool cnn_wgrad_lds_shape(int cin, int cout) {
-    return cout == 16 && (cin == 8 || cin == 16 || cin == 48);
-}
-static bool wgrad_lds_shape(int cin, int cout) { return cnn_wgrad_lds_shape(cin, cout); }
 struct WgradGrid {
     int tiles_x, tiles_y, zc, groups;
 };
-static WgradGrid wgrad_lds_grid(int cin, int nz, int ny, int nx) {
+static WgradGrid wgrad_lds_grid(int cin, int cout, int nz, int ny, int nx) {
     WgradGrid g;
-    g.tiles_x = (nx + WG_XW - 1) / WG_XW;
-    g.tiles_y = (ny + wgrad_ty(cin) - 1) / wgrad_ty(cin);
-    // about 256 groups (one per CU), each marching over zc slices
+    const int xw = wgrad_xw(cin), ty = wgrad_ty(cin, cout);
+    g.tiles_x = (nx + xw - 1) / xw;
+    g.tiles_y = (ny + ty - 1) / ty;
+    // about 256 spatial groups (one per CU), each marching over zc slices
     const int64_t tiles = (int64_t)g.tiles_x * g.tiles_y;
     int64_t nzc = (256 + tiles - 1) / tiles;
     if (nzc > nz) nzc = nz;
@@ -1191,99 +988,51 @@ static WgradGrid wgrad_lds_grid(int cin, int nz, int ny, int nx) {
     return g;
 }
 static size_t wgrad_lds_bytes(int cin, int cout, int nz, int ny, int nx) {
-    const WgradGrid g = wgrad_lds_grid(cin, nz, ny, nx);
+    const WgradGrid g = wgrad_lds_grid(cin, cout, nz, ny, nx);
     return sizeof(float) * (size_t)g.groups * ((size_t)cout * 27 * cin + cout) + 256;
 }
 
-size_t cnn_wgrad_work_bytes(int cin, int cout, int nz, int ny, int nx, int nsplit) {
-    const int64_t plane = (int64_t)nz * ny * cnn_nxp(nx);
-    const int64_t ci16 = (cin + 15) / 16 * 16;
-    const size_t generic =
-        sizeof(u16) * (size_t)(plane * cout + 3 * plane * ci16) +
-        sizeof(float) * ((size_t)nsplit * cout * 27 * cin + (size_t)BIAS_SPLIT * cout) + 256;
-    return wgrad_lds_shape(cin, cout) ? wgrad_lds_bytes(cin, cout, nz, ny, nx) : generic;
+size_t cnn_wgrad_work_bytes(int cin, int cout, int nz, int ny, int nx) {
+    return wgrad_lds_bytes(cin, cout, nz, ny, nx);
 }
 
-template <int CIN>
+template <int CIN, int COUT>
 static hipError_t wgrad_lds_t(const void* x, int xcs, int xco, const void* dy, int dcs, int dco,
                               int nz, int ny, int nx, float* dw, float* db, void* work,
                               hipStream_t st) {
-    const WgradGrid g = wgrad_lds_grid(CIN, nz, ny, nx);
-    const int64_t n = (int64_t)16 * 27 * CIN;
+    const WgradGrid g = wgrad_lds_grid(CIN, COUT, nz, ny, nx);
+    const int64_t n = (int64_t)COUT * 27 * CIN;
     float* part = (float*)(((uintptr_t)work + 255) & ~(uintptr_t)255);
     float* bpart = part + (int64_t)g.groups * n;
-    const size_t lds = (size_t)(3 * wgrad_slab(CIN) + 2 * wgrad_dslab(CIN)) * 16;
-    const auto kern = wgrad_lds_kernel<CIN>;
+    const size_t lds = (size_t)(3 * wgrad_slab(CIN, COUT) + 2 * wgrad_dslab(CIN, COUT)) * 16;
+    const auto kern = wgrad_lds_kernel<CIN, COUT>;
     if (lds > 65536) {
         const hipError_t e = hipFuncSetAttribute(
             (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3((unsigned)g.groups), dim3(256), lds, st, (const u16*)x, xcs,
-                       xco, (const u16*)dy, dcs, dco, part, bpart, nz, ny, nx, g.tiles_x,
-                       g.tiles_y, g.zc);
+    const dim3 grid((unsigned)g.groups, (unsigned)(COUT / 16 * wgrad_shares(CIN)));
+    hipLaunchKernelGGL(kern, grid, dim3(256), lds, st, (const u16*)x, xcs, xco, (const u16*)dy,
+                       dcs, dco, part, bpart, nz, ny, nx, g.tiles_x, g.tiles_y, g.zc);
     hipLaunchKernelGGL(sum_parts_kernel, dim3(sum_parts_grid(n)), dim3(256), 0, st,
                        (const float*)part, g.groups, n, dw);
     if (db)
-        hipLaunchKernelGGL(sum_parts_kernel, dim3(sum_parts_grid(16)), dim3(256), 0, st,
-                           (const float*)bpart, g.groups, (int64_t)16, db);
+        hipLaunchKernelGGL(sum_parts_kernel, dim3(sum_parts_grid(COUT)), dim3(256), 0, st,
+                           (const float*)bpart, g.groups, (int64_t)COUT, db);
     return hipGetLastError();
 }
 
 hipError_t launch_cnn_wgrad(int cin, int cout, const void* x, int xcs, int xco, const void* dy,
                             int dcs, int dco, int nz, int ny, int nx, float* dw, float* db,
-                            void* work, int nsplit, hipStream_t st) {
-    if (wgrad_lds_shape(cin, cout)) {
-        if (xcs % 8 || xco % 8 || dcs % 8 || dco % 8 || (uintptr_t)x % 16 || (uintptr_t)dy % 16)
-            return hipErrorInvalidValue;
-        if (cin == 8) return wgrad_lds_t<8>(x, xcs, xco, dy, dcs, dco, nz, ny, nx, dw, db, work, st);
-        if (cin == 16)
-            return wgrad_lds_t<16>(x, xcs, xco, dy, dcs, dco, nz, ny, nx, dw, db, work, st);
-        return wgrad_lds_t<48>(x, xcs, xco, dy, dcs, dco, nz, ny, nx, dw, db, work, st);
-    }
-    const int nxp = cnn_nxp(nx);
-    const int64_t plane = (int64_t)nz * ny * nxp;
-    const int ci16 = (cin + 15) / 16 * 16;
-    u16* dy_cm = (u16*)work;
-    u16* xs_cm = dy_cm + plane * cout;
-    float* part = (float*)(((uintptr_t)(xs_cm + 3 * plane * ci16) + 255) & ~(uintptr_t)255);
-    const int64_t rows = (int64_t)nz * ny;
-    const bool tiled = cout % 8 == 0 && dco % 8 == 0 && dcs % 8 == 0 && cout <= TO_CM_MAX_C &&
-                       cin % 8 == 0 && xco % 8 == 0 && xcs % 8 == 0 && ci16 <= TO_CM_MAX_C &&
-                       rows * (nxp / 32) <= 0x7fffffff;
-    if (tiled) {
-        const dim3 grid((unsigned)(rows * (nxp / 32)));
-        hipLaunchKernelGGL(to_cm_tile_kernel<1>, grid, dim3(256), 0, st, (const u16*)dy, dcs,
-                           dco, cout, cout, dy_cm, nz, ny, nx, nxp);
-        hipLaunchKernelGGL(to_cm_tile_kernel<3>, grid, dim3(256), 0, st, (const u16*)x, xcs, xco,
-                           cin, ci16, xs_cm, nz, ny, nx, nxp);
-    } else {
-        hipLaunchKernelGGL(to_cm_kernel, dim3(grid_for(plane * cout)), dim3(256), 0, st,
-                           (const u16*)dy, dcs, dco, cout, cout, dy_cm, nz, ny, nx, nxp, 0);
-        for (int s = 0; s < 3; ++s)
-            hipLaunchKernelGGL(to_cm_kernel, dim3(grid_for(plane * ci16)), dim3(256), 0, st,
-                               (const u16*)x, xcs, xco, cin, ci16,
-                               xs_cm + (int64_t)s * ci16 * plane, nz, ny, nx, nxp, s - 1);
-    }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-#define CASE(A, B) \
-    if (cin == A && cout == B) e = wgrad_t<A, B>(dy_cm, xs_cm, part, nsplit, nz, ny, nxp, st); else
+                            void* work, hipStream_t st) {
+    if (xcs % 8 || xco % 8 || dcs % 8 || dco % 8 || (uintptr_t)x % 16 || (uintptr_t)dy % 16)
+        return hipErrorInvalidValue;
+#define CASE(A, B)      \
+    if (cin == A && cout == B) \
+        return wgrad_lds_t<A, B>(x, xcs, xco, dy, dcs, dco, nz, ny, nx, dw, db, work, st);
     OSTR_CNN_WGRAD_SHAPES(CASE)
 #undef CASE
     return hipErrorInvalidValue;
-    if (e != hipSuccess) return e;
-    const int64_t n = (int64_t)cout * 27 * cin;
-    hipLaunchKernelGGL(sum_parts_kernel, dim3(sum_parts_grid(n)), dim3(256), 0, st,
-                       (const float*)part, nsplit, n, dw);
-    if (db) {
-        float* bpart = part + (int64_t)nsplit * n;
-        hipLaunchKernelGGL(bias_grad_kernel, dim3(cout, BIAS_SPLIT), dim3(256), 0, st,
-                           (const u16*)dy_cm, plane, bpart);
-        hipLaunchKernelGGL(sum_parts_kernel, dim3(sum_parts_grid(cout)), dim3(256), 0, st,
-                           (const float*)bpart, BIAS_SPLIT, (int64_t)cout, db);
-    }
-    return hipGetLastError();
 }
 
 // 16-byte access allowed: channel count, strides, offsets and pointers, and

@@ -317,18 +317,15 @@ hipError_t launch_batch_update(const BatchCase* cases, const int* sched, int n_c
 // stride, channel offset).
 bool cnn_conv_supported(int cin, int cout);
 bool cnn_wgrad_supported(int cin, int cout);
-// (cin, cout) whose weight gradient runs through LDS (views and pointers aligned)
-bool cnn_wgrad_lds_shape(int cin, int cout);
 int cnn_kpad(int c);
-int cnn_nxp(int nx);
-size_t cnn_wgrad_work_bytes(int cin, int cout, int nz, int ny, int nx, int nsplit);
+size_t cnn_wgrad_work_bytes(int cin, int cout, int nz, int ny, int nx);
 int64_t cnn_mse_partials(int64_t n);
 hipError_t launch_cnn_conv(int cin, int cout, const void* x, int xcs, int xco, const void* w,
                            const float* bias, void* y, int ycs, int yco, const void* mask,
                            int mcs, int mco, int relu, int nz, int ny, int nx, hipStream_t st);
 hipError_t launch_cnn_wgrad(int cin, int cout, const void* x, int xcs, int xco, const void* dy,
                             int dcs, int dco, int nz, int ny, int nx, float* dw, float* db,
-                            void* work, int nsplit, hipStream_t st);
+                            void* work, hipStream_t st);
 hipError_t launch_cnn_pool(const void* x, int xcs, int xco, void* y, int ycs, int yco, int C,
                            int nz, int ny, int nx, hipStream_t st);
 hipError_t launch_cnn_pool_bwd(const void* dy, int dcs, int dco, void* dx, int xcs, int xco,

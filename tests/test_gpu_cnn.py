@@ -122,19 +122,21 @@ def test_conv3d_z_march_matches_fp64(dev, cin, cout):
         assert rel_l2(got[..., yco:yco + cout], want) <= 1e-2
 
 
-@pytest.mark.parametrize("cin", [8, 16, 48])
-def test_weight_gradient_z_march_matches_fp64(dev, cin):
-    """The weight and bias gradients of the full-resolution shapes (cin -> 16)
-    at 13 x 37 x 150 voxels: work-groups march over 1-2 slices, the last x, y
-    and z tiles are partial, x and dY are channel views (offset 8 in wider
-    buffers). The products of BF16 values are exact in FP32, so the only
-    error is the FP32 accumulation over 72k voxels. Bar: relative L2 <= 1e-4
-    (weights and bias) against FP64, and the same bits on a repeat."""
+@pytest.mark.parametrize("cin,cout", [(8, 16), (16, 16), (48, 16), (16, 32), (32, 32),
+                                      (32, 64), (64, 64), (96, 32)])
+def test_weight_gradient_z_march_matches_fp64(dev, cin, cout):
+    """The weight and bias gradients of every network shape at 13 x 37 x 150
+    voxels: work-groups march over 1-2 slices, the last x, y and z tiles are
+    partial, x and dY are channel views (offset 8 in wider buffers); the wider
+    shapes split their output channels and accumulator tiles over groups. The
+    products of BF16 values are exact in FP32, so the only error is the FP32
+    accumulation over 72k voxels. Bar: relative L2 <= 1e-4 (weights and bias)
+    against FP64, and the same bits on a repeat."""
     net = W.WarmStartNet(dev)
     L = net._L
-    rng = np.random.default_rng(40 + cin)
-    nz, ny, nx, cout = 13, 37, 150, 16
-    xcs, xco, dcs, dco = cin + 8, 8, 32, 8
+    rng = np.random.default_rng(40 + cin + cout)
+    nz, ny, nx = 13, 37, 150
+    xcs, xco, dcs, dco = cin + 8, 8, cout + 16, 8
     x = bf16_round(rng.standard_normal((nz, ny, nx, cin)))
     gy = bf16_round(rng.standard_normal((nz, ny, nx, cout)))
     w = np.zeros((cout, 27, cin))
