@@ -11,8 +11,8 @@
 // q = IFFT(FFT(pw column) FFT(H)). Two real columns ride in one complex
 // sequence (column j real, j + 1 imaginary): H is real, so the convolution
 // keeps them apart. Per column pair a forward and an inverse radix-2 FFT
-// (L/2 log2 L butterflies each) replace the 2 nbx^2 multiply-adds of the
-// direct sum.
+// (L/2 log2 L butterflies each, two stages per LDS pass) replace the 2 nbx^2
+// multiply-adds of the direct sum.
 //
 // fbp_spectrum_kernel (one group) forms FFT(H) / L and the twiddle table
 // exp(-2 pi i k / L), k < L/2, rounded from double; fbp_filter_kernel gives
@@ -44,43 +44,95 @@ __device__ __forceinline__ unsigned bitrev(unsigned i, int logl) {
 }
 
 // Forward transform, bit-reversed order in, natural order out, on the seq
-// sequences of L = 2^logl elements at buf + s L. Ends with a barrier.
+// sequences of L = 2^logl elements at buf + s L. Radix-2 decimation in time,
+// two stages at a time: the quadruple (i0, i0 + m, i0 + 2m, i0 + 3m) of stage
+// half-size m goes through both radix-2 butterflies in registers (the same
+// operations, one LDS round trip and barrier instead of two); an odd first
+// stage runs alone. Ends with a barrier.
 __device__ void fft_dit(float2* buf, int seq, int logl, const float2* tw) {
     const int L = 1 << logl;
-    for (int lg = 0; lg < logl; ++lg) {
-        const int m = 1 << lg;
+    int lg = 0;
+    if (logl & 1) {
         __syncthreads();
         for (int b = threadIdx.x; b < L / 2; b += FFT_THREADS) {
-            const int k = b & (m - 1);
-            const int i0 = ((b >> lg) << (lg + 1)) | k;
-            const float2 w = tw[k << (logl - 1 - lg)];
+            const int i0 = b << 1;
             for (int s = 0; s < seq; ++s) {
                 float2* a = buf + s * L;
-                const float2 u = a[i0], o = cmul(a[i0 + m], w);
+                const float2 u = a[i0], o = cmul(a[i0 + 1], tw[0]);
                 a[i0] = cadd(u, o);
-                a[i0 + m] = csub(u, o);
+                a[i0 + 1] = csub(u, o);
+            }
+        }
+        lg = 1;
+    }
+    for (; lg < logl; lg += 2) {
+        const int m = 1 << lg;
+        __syncthreads();
+        for (int b = threadIdx.x; b < L / 4; b += FFT_THREADS) {
+            const int k = b & (m - 1);
+            const int i0 = ((b >> lg) << (lg + 2)) | k;
+            const float2 w = tw[k << (logl - 1 - lg)];
+            const float2 w4a = tw[k << (logl - 2 - lg)];
+            const float2 w4b = tw[(k + m) << (logl - 2 - lg)];
+            for (int s = 0; s < seq; ++s) {
+                float2* a = buf + s * L;
+                const float2 e0 = a[i0], e1 = a[i0 + m], e2 = a[i0 + 2 * m], e3 = a[i0 + 3 * m];
+                const float2 o1 = cmul(e1, w), o3 = cmul(e3, w);
+                const float2 A0 = cadd(e0, o1), A1 = csub(e0, o1);
+                const float2 A2 = cadd(e2, o3), A3 = csub(e2, o3);
+                const float2 p2 = cmul(A2, w4a), p3 = cmul(A3, w4b);
+                a[i0] = cadd(A0, p2);
+                a[i0 + 2 * m] = csub(A0, p2);
+                a[i0 + m] = cadd(A1, p3);
+                a[i0 + 3 * m] = csub(A1, p3);
             }
         }
     }
     __syncthreads();
 }
 
-// Inverse transform without the 1/L, natural order in, bit-reversed order out.
+// Inverse transform without the 1/L, natural order in, bit-reversed order out:
+// radix-2 decimation in frequency with conjugate twiddles, two stages at a
+// time (the quadruple (i0, i0 + M/2, i0 + M, i0 + 3M/2) of stage half-size M
+// through both butterflies in registers); an odd last stage runs alone.
 __device__ void fft_dif_inverse(float2* buf, int seq, int logl, const float2* tw) {
     const int L = 1 << logl;
-    for (int lg = logl - 1; lg >= 0; --lg) {
-        const int m = 1 << lg;
+    auto ctw = [&](int i) {
+        const float2 t = tw[i];
+        return make_float2(t.x, -t.y);
+    };
+    int lg = logl - 1;
+    for (; lg >= 1; lg -= 2) {
+        const int M = 1 << lg, h = M >> 1;
         __syncthreads();
-        for (int b = threadIdx.x; b < L / 2; b += FFT_THREADS) {
-            const int k = b & (m - 1);
-            const int i0 = ((b >> lg) << (lg + 1)) | k;
-            const float2 t = tw[k << (logl - 1 - lg)];
-            const float2 w = make_float2(t.x, -t.y);
+        for (int b = threadIdx.x; b < L / 4; b += FFT_THREADS) {
+            const int k = b & (h - 1);
+            const int i0 = ((b >> (lg - 1)) << (lg + 1)) | k;
+            const float2 wk = ctw(k << (logl - 1 - lg));
+            const float2 wh = ctw((k + h) << (logl - 1 - lg));
+            const float2 v = ctw(k << (logl - lg));
             for (int s = 0; s < seq; ++s) {
                 float2* a = buf + s * L;
-                const float2 u = a[i0], v = a[i0 + m];
+                const float2 e0 = a[i0], e1 = a[i0 + h], e2 = a[i0 + M], e3 = a[i0 + M + h];
+                const float2 A0 = cadd(e0, e2), A2 = cmul(csub(e0, e2), wk);
+                const float2 A1 = cadd(e1, e3), A3 = cmul(csub(e1, e3), wh);
+                a[i0] = cadd(A0, A1);
+                a[i0 + h] = cmul(csub(A0, A1), v);
+                a[i0 + M] = cadd(A2, A3);
+                a[i0 + M + h] = cmul(csub(A2, A3), v);
+            }
+        }
+    }
+    if (lg == 0) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < L / 2; b += FFT_THREADS) {
+            const int i0 = b << 1;
+            const float2 w = ctw(0);
+            for (int s = 0; s < seq; ++s) {
+                float2* a = buf + s * L;
+                const float2 u = a[i0], v = a[i0 + 1];
                 a[i0] = cadd(u, v);
-                a[i0 + m] = cmul(csub(u, v), w);
+                a[i0 + 1] = cmul(csub(u, v), w);
             }
         }
     }
