@@ -147,6 +147,53 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
             const bool wave_i_ok = __all(i0 && i1);
             const float omfi = 1.0f - fi;
             const uint32_t base = (uint32_t)a * nbin + (uint32_t)il;
+            // Common case: every lane's two columns and every row's two rows
+            // on the detector. Then the taps of all ROWS rows are loaded in
+            // one straight run (2 ROWS loads in flight) before any is used;
+            // with a per-row choice of load form each row's loads would wait
+            // for the previous row's. Same operations in the same order as
+            // the per-row form below, so the same bits.
+            bool all_rows = wave_i_ok;
+            uint32_t roff[ROWS];
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) {
+                const int jl = __builtin_amdgcn_readlane(rt_jl, k * ROWS + r);
+                all_rows = all_rows && (unsigned)jl < (unsigned)g.nby &&
+                           (unsigned)(jl + 1) < (unsigned)g.nby &&
+                           (sizeof(TP) == 4 || a + 1 < n || jl + 2 < g.nby);
+                roff[r] = base + (uint32_t)jl * nbx;
+            }
+            if (all_rows) {
+                float p[ROWS][4];
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r) {
+                    P.pair(roff[r], p[r][0], p[r][1]);
+                    P.pair(roff[r] + nbx, p[r][2], p[r][3]);
+                }
+#pragma unroll
+                for (int r = 0; r < ROWS; ++r) {
+                    const int src = k * ROWS + r;
+                    const float fj = readlane_f(rt_fj, src);
+                    const float ey = readlane_f(rt_ey, src);
+                    const float den = fmaf(ez, ez, fmaf(ey, ey, exx));
+                    const float dw = num * __builtin_amdgcn_rcpf(den);
+                    float d1 = omfi * p[r][0];
+                    d1 = fmaf(fi, p[r][1], d1);
+                    float d2 = omfi * p[r][2];
+                    d2 = fmaf(fi, p[r][3], d2);
+                    float u = fmaf(fj, d2, (1.0f - fj) * d1);
+                    u = (u * g.dz) * dw;
+                    total[r] = total[r] + u;
+                    if (SENS) {
+                        // every tap valid: d1_sen = d2_sen = (1-fI) + fI
+                        const float d1s = omfi + fi;
+                        float us = fmaf(fj, d1s, (1.0f - fj) * d1s);
+                        us = (us * g.dz) * dw;
+                        sens[r] = sens[r] + us;
+                    }
+                }
+                continue;
+            }
 #pragma unroll
             for (int r = 0; r < ROWS; ++r) {
                 const int src = k * ROWS + r;
