@@ -548,11 +548,7 @@ __global__ __launch_bounds__(256) void update_fused_matched_kernel(
     const TI* __restrict__ x_in, TI* __restrict__ x_out, const float* __restrict__ d,
     PriorClass W, float inv_delta, float beta, float capl_scale, int guard) {
     constexpr int R = MATCHED_ROWS;
-    __shared__ MatchedTab T;
     const TileCoord c = tile_coord<R, CASE_WAVES>(g, xcd_tile(blockIdx.x, gridDim.x));
-    // the group's x-slab reciprocals (all waves help, before any leaves)
-    matched_tab_build(g, S, n_slots, c.ix - (int)(threadIdx.x & 63), c.iz, T);
-    __syncthreads();
     if (c.iy0 >= g.ny) return;  // see sbp_kernel
     const uint32_t nvox = (uint32_t)g.nx * g.ny * g.nz;
     const VolReader<TP> P(hsub, (uint32_t)n_slots * g.nbx * g.nby);
@@ -563,7 +559,7 @@ __global__ __launch_bounds__(256) void update_fused_matched_kernel(
 #pragma unroll
     for (int r = 0; r < R; ++r)
         total[r] = (c.ix < g.nx && c.iy0 + r < g.ny)
-                       ? matched_voxel<TP>(g, S, n_slots, 1, P, c.ix, c.iy0 + r, c.iz, &T)
+                       ? matched_voxel<TP>(g, S, n_slots, 1, P, c.ix, c.iy0 + r, c.iz)
                        : 0.0f;
     by_prior_mode<R>(g, c, W, [&](auto m) {
         update_rows<POT, decltype(m)::value, TI, R>(g, X, XO, D, c, total, W, inv_delta,
