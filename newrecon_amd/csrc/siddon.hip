@@ -25,7 +25,7 @@ namespace {
 
 constexpr int TILE_I = 64;   // bins along detector x per tile (one per lane)
 constexpr int TILE_J = 4;    // bins along detector y per tile (one per wave)
-constexpr int WALK_BATCH = 4;  // walk steps whose gathers are in flight together
+constexpr int WALK_BATCH = 4;  // walk steps per batch (two batches' gathers in flight)
 
 __device__ __forceinline__ float bin_centre(float origin, int i, float size) {
     return (float)((double)origin + ((double)i + 0.5) * (double)size);
@@ -177,13 +177,12 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
         // accumulated in step order, so a batch pays one memory latency.
         const float thr = stop_threshold(lmax);
         bool active = at < thr;
-        while (active) {
-            // branch-free: a lane that stops inside the batch keeps its state
-            // through the remaining steps (every update is a select on act),
-            // so the per-step flags stay lane masks and no step needs a branch
-            float seg[WALK_BATCH];
-            uint32_t off[WALK_BATCH];
-            bool ok[WALK_BATCH];
+        // One batch of steps: its geometry into seg / off / ok. Branch-free: a
+        // lane that stops inside the batch keeps its state through the
+        // remaining steps (every update is a select on act), so the per-step
+        // flags stay lane masks and no step needs a branch.
+        auto batch = [&](float (&seg)[WALK_BATCH], uint32_t (&off)[WALK_BATCH],
+                         bool (&ok)[WALK_BATCH]) {
 #pragma unroll
             for (int k = 0; k < WALK_BATCH; ++k) {
                 const bool act = active;
@@ -209,15 +208,48 @@ __device__ __forceinline__ float walk(const DevGeom& g, const VolReader<TI>& img
                 at = act ? nat : at;
                 active = act & (at < thr);
             }
-            float v[WALK_BATCH];
+        };
+        auto gather = [&](const uint32_t (&off)[WALK_BATCH], float (&v)[WALK_BATCH]) {
 #pragma unroll
             for (int k = 0; k < WALK_BATCH; ++k) v[k] = ONES ? 1.0f : img.at_byte(off[k]);
+        };
+        auto accumulate = [&](const float (&v)[WALK_BATCH], const float (&seg)[WALK_BATCH],
+                              const bool (&ok)[WALK_BATCH]) {
 #pragma unroll
             for (int k = 0; k < WALK_BATCH; ++k) {
                 sum = (ONES && !ok[k]) ? sum : fmaf(v[k], seg[k], sum);
                 if (NORM) *norm = ok[k] ? fmaf(1.0f, seg[k], *norm) : *norm;
                 if (COUNT && ok[k] && seg[k] > 0.0f) ++nseg;
             }
+        };
+        // Software-pipelined over two sets of batch registers (A, B): the
+        // gathers of the next batch are issued before the values of the
+        // current one are summed, so their latency overlaps the next batch's
+        // geometry (the scheduling barriers keep the compiler from hoisting a
+        // sum, and with it the wait for its gathers, above that geometry).
+        // The wave runs while any of its lanes walks, two batches per turn; a
+        // lane that has stopped takes masked steps (state held, gathers past
+        // the buffer's end, which add nothing), so its batches and the order
+        // of its sums are those of a plain batch loop.
+        if (active) {
+            float segA[WALK_BATCH], vA[WALK_BATCH], segB[WALK_BATCH], vB[WALK_BATCH];
+            uint32_t offA[WALK_BATCH], offB[WALK_BATCH];
+            bool okA[WALK_BATCH], okB[WALK_BATCH];
+            batch(segA, offA, okA);
+            gather(offA, vA);
+            while (true) {
+                batch(segB, offB, okB);
+                gather(offB, vB);
+                __builtin_amdgcn_sched_barrier(0);
+                accumulate(vA, segA, okA);
+                __builtin_amdgcn_sched_barrier(0);
+                batch(segA, offA, okA);
+                gather(offA, vA);
+                __builtin_amdgcn_sched_barrier(0);
+                accumulate(vB, segB, okB);
+                if (!__any(active)) break;
+            }
+            accumulate(vA, segA, okA);
         }
     } else {
         // lmax <= 0 (volume behind the source): the literal relative test
