@@ -154,21 +154,34 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
             // for the previous row's. Same operations in the same order as
             // the per-row form below, so the same bits.
             bool all_rows = wave_i_ok;
-            uint32_t roff[ROWS];
+            int jlr[ROWS];  // wave-uniform
 #pragma unroll
             for (int r = 0; r < ROWS; ++r) {
                 const int jl = __builtin_amdgcn_readlane(rt_jl, k * ROWS + r);
                 all_rows = all_rows && (unsigned)jl < (unsigned)g.nby &&
                            (unsigned)(jl + 1) < (unsigned)g.nby &&
                            (sizeof(TP) == 4 || a + 1 < n || jl + 2 < g.nby);
-                roff[r] = base + (uint32_t)jl * nbx;
+                jlr[r] = jl;
             }
             if (all_rows) {
                 float p[ROWS][4];
+                if constexpr (sizeof(TP) == 4) {
+                    // the row's offset jl nbx is wave-uniform: it rides in the
+                    // scalar offset, the lane's column in the vector offset
+                    const uint32_t vb = base * 4u;
 #pragma unroll
-                for (int r = 0; r < ROWS; ++r) {
-                    P.pair(roff[r], p[r][0], p[r][1]);
-                    P.pair(roff[r] + nbx, p[r][2], p[r][3]);
+                    for (int r = 0; r < ROWS; ++r) {
+                        const uint32_t jl = (uint32_t)jlr[r];
+                        P.pair_bytes(vb, jl * nbx * 4u, p[r][0], p[r][1]);
+                        P.pair_bytes(vb, (jl + 1u) * nbx * 4u, p[r][2], p[r][3]);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < ROWS; ++r) {
+                        const uint32_t off = base + (uint32_t)jlr[r] * nbx;
+                        P.pair(off, p[r][0], p[r][1]);
+                        P.pair(off + nbx, p[r][2], p[r][3]);
+                    }
                 }
 #pragma unroll
                 for (int r = 0; r < ROWS; ++r) {

@@ -185,6 +185,11 @@ struct VolReader {
     // 2-byte types from the 4-byte boundary at or below e, so when e is odd
     // element e + 2 must lie inside the buffer (the caller guarantees it)
     __device__ __forceinline__ void pair(uint32_t e, float& a, float& b) const;
+    // pair() of a 4-byte type at byte offset vb + sb, with sb wave-uniform (it
+    // goes in the load's scalar offset, so the lane's address costs nothing
+    // per row)
+    __device__ __forceinline__ void pair_bytes(uint32_t vb, uint32_t sb, float& a,
+                                               float& b) const;
 };
 template <>
 __device__ __forceinline__ float VolReader<float>::at_byte(uint32_t b) const {
@@ -209,6 +214,13 @@ __device__ __forceinline__ uint32_t pair16(__amdgpu_buffer_rsrc_t rsrc, uint32_t
 template <>
 __device__ __forceinline__ void VolReader<float>::pair(uint32_t e, float& a, float& b) const {
     const u32x2_t w = __builtin_amdgcn_raw_buffer_load_b64(rsrc, e * 4u, 0, 0);
+    a = __uint_as_float(w.x);
+    b = __uint_as_float(w.y);
+}
+template <>
+__device__ __forceinline__ void VolReader<float>::pair_bytes(uint32_t vb, uint32_t sb, float& a,
+                                                             float& b) const {
+    const u32x2_t w = __builtin_amdgcn_raw_buffer_load_b64(rsrc, vb, sb, 0);
     a = __uint_as_float(w.x);
     b = __uint_as_float(w.y);
 }
