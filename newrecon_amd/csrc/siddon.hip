@@ -287,6 +287,16 @@ __device__ __forceinline__ void project_bin(const DevGeom& g, const ViewTab& v, 
                                             TO* __restrict__ out, const float* __restrict__ y,
                                             const float* __restrict__ b,
                                             const float* __restrict__ r) {
+    const int64_t o_ref = ((int64_t)slot * g.nbx + i) * g.nby + j;
+    const int64_t o_row = ((int64_t)slot * g.nby + j) * g.nbx + i;
+    // the h epilogue's inputs are loaded first: their latency overlaps the
+    // ray set-up and the walk (most bins miss the volume and have no walk)
+    float yb = 0.0f, bb = 0.0f, rb = 0.0f;
+    if (KIND == 3) {
+        yb = y[o_row];
+        bb = b[o_row];
+        if (r) rb = r[o_row];
+    }
     const float ex = bin_centre(g.x_d0, i, g.dbx);
     const float ey = bin_centre(g.y_d0, j, g.dby);
     const float ez = -g.det_r;
@@ -312,13 +322,11 @@ __device__ __forceinline__ void project_bin(const DevGeom& g, const ViewTab& v, 
         val = s * v.c;
     else
         val = s;
-    const int64_t o_ref = ((int64_t)slot * g.nbx + i) * g.nby + j;
-    const int64_t o_row = ((int64_t)slot * g.nby + j) * g.nbx + i;
     if (KIND == 3) {
         // h epilogue (compute_h, Tomo_OSTR_gold.cpp:255-258)
-        float yt = b[o_row] * expf(-val);
-        float den = r ? yt + r[o_row] : yt;
-        val = (y[o_row] / den - 1.0f) * yt;
+        float yt = bb * expf(-val);
+        float den = r ? yt + rb : yt;
+        val = (yb / den - 1.0f) * yt;
     }
     if (KIND == 4 || KIND == 5) {
         // yry * gamma (compute_yry :183-205, compute_gamma_yry :208-218)
