@@ -10,8 +10,10 @@
 // from the source onto the detector plane; each candidate is clipped against
 // the voxel box (slab method: the z-slab clip once per slot, the y slab once
 // per detector row, the x slab per bin; min / max are order-free, so the
-// clip is that of the three slabs taken in any order; the ray length uses the
-// hardware square root, <= 1 ulp). The sinogram is detector-row-major
+// clip is that of the three slabs taken in any order; the slab reciprocals
+// and the ray length use the hardware reciprocal and square root, <= 1 ulp
+// each: the gather is bound by its clip arithmetic, and an exact division per
+// slab cost a quarter of its time). The sinogram is detector-row-major
 // [slot][nby][nbx], so neighbouring lanes (voxels along x) read neighbouring
 // bins.
 //
@@ -38,7 +40,7 @@ __device__ __forceinline__ void matched_slab(float s, float e, float lo, float h
                                              float& t1) {
     const float d = e - s;
     if (d != 0.0f) {
-        const float inv = 1.0f / d;
+        const float inv = __builtin_amdgcn_rcpf(d);
         float a = (lo - s) * inv, b = (hi - s) * inv;
         t0 = fmaxf(t0, fminf(a, b));
         t1 = fminf(t1, fmaxf(a, b));
