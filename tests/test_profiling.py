@@ -68,4 +68,6 @@ def test_committed_summaries_name_the_bench_kernels():
         for name in bench.PROFILED_KERNEL.values():
             assert name in kernels, (os.path.basename(path), name)
     issue, busy = bench.profiled_valu("update_fused")
-    assert 0.0 < issue <= busy <= 1.05
+    # busy counts the second issue slot of rsq / rcp and the division helpers,
+    # so it can pass 1 (never 2: every instruction takes at most two slots)
+    assert 0.0 < issue <= busy < 2.0
