@@ -31,8 +31,14 @@ struct MatchedSlots {
     float x0[MAX_SLOTS], z0[MAX_SLOTS], c[MAX_SLOTS];
 };
 
+// origin + (i + 1/2) size rounded once to float: i + 1/2 is exact in FP32
+// (|i| < 2^23), so the fma is the correctly rounded value. The forward walk's
+// bin_centre forms the same sum in FP64 and rounds it to float; the product
+// is exact in FP64 (at most 48 significant bits) and so is the sum while
+// |origin| < 2^28 size (all its bits within 53 places), so the two agree, at
+// a quarter of the cost (no FP64 conversions or arithmetic).
 __device__ __forceinline__ float matched_bin_centre(float origin, int i, float size) {
-    return (float)((double)origin + ((double)i + 0.5) * (double)size);
+    return fmaf((float)i + 0.5f, size, origin);
 }
 
 // Parametric overlap of the line s + t (e - s) with [lo, hi] along one axis.
