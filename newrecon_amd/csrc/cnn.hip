@@ -567,8 +567,19 @@ __global__ __launch_bounds__(256) void sum_parts_kernel(const float* __restrict_
     const int il = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int64_t i = (int64_t)blockIdx.x * 64 + il;
     float s = 0.0f;
-    if (i < n)
-        for (int k = g; k < nsplit; k += 4) s += part[(int64_t)k * n + i];
+    if (i < n) {
+        // eight partials loaded before they are added (in the same order),
+        // so a lane waits for one memory round trip per eight, not per term
+        int k = g;
+        for (; k + 28 < nsplit; k += 32) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = part[(int64_t)(k + 4 * j) * n + i];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s += v[j];
+        }
+        for (; k < nsplit; k += 4) s += part[(int64_t)k * n + i];
+    }
     red[g][il] = s;
     __syncthreads();
     if (g == 0 && i < n) dw[i] = ((red[0][il] + red[1][il]) + red[2][il]) + red[3][il];
