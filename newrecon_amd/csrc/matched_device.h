@@ -109,15 +109,28 @@ __device__ __forceinline__ float matched_voxel(const DevGeom& g, const MatchedSl
             if (!(ty0 < ty1)) continue;
             const float ddy = ey - sy;
             const float dyz = fmaf(ddz, ddz, ddy * ddy);
-            for (int i = i0; i <= i1; ++i) {
-                const float ex = matched_bin_centre(g.x_d0, i, g.dbx);
+            // the column as an FP32 counter (i + 1/2 exact), so the bin centre
+            // is matched_bin_centre's fma without a conversion; the x slab of
+            // matched_slab inline, its d = sx - ex also giving the ray
+            // length (d^2 = (ex - sx)^2 exactly)
+            const uint32_t row = (uint32_t)a * nbin + (uint32_t)j * g.nbx;
+            float fi = (float)i0 + 0.5f;
+            for (int i = i0; i <= i1; ++i, fi += 1.0f) {
+                const float ex = fmaf(fi, g.dbx, g.x_d0);
                 float t0 = ty0, t1 = ty1;
-                matched_slab(ex, sx, bx0, bx1, t0, t1);
+                const float d = sx - ex;
+                if (d != 0.0f) {
+                    const float inv = __builtin_amdgcn_rcpf(d);
+                    const float c0 = (bx0 - ex) * inv, c1 = (bx1 - ex) * inv;
+                    t0 = fmaxf(t0, fminf(c0, c1));
+                    t1 = fminf(t1, fmaxf(c0, c1));
+                } else if (!(ex > bx0 && ex < bx1)) {
+                    t1 = -1.0f;  // parallel and outside: empty
+                    t0 = 1.0f;
+                }
                 if (t0 < t1) {
-                    const float ddx = ex - sx;
-                    const float len = __builtin_amdgcn_sqrtf(fmaf(ddx, ddx, dyz));
-                    acc = fmaf((t1 - t0) * len, P((uint32_t)a * nbin + (uint32_t)j * g.nbx + i),
-                               acc);
+                    const float len = __builtin_amdgcn_sqrtf(fmaf(d, d, dyz));
+                    acc = fmaf((t1 - t0) * len, P(row + (uint32_t)i), acc);
                 }
             }
         }
