@@ -115,7 +115,7 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
     }
     for (int c0 = 0; c0 < n; c0 += VIEW_CHUNK) {
         int rt_jl = -2;
-        float rt_fj = 0.0f, rt_ey = 0.0f;
+        float rt_fj = 0.0f, rt_ey = 0.0f, rt_t = 0.0f;
         {
             const int r = lane % ROWS, a = c0 + lane / ROWS;
             const float z0 = __shfl(sz0, a & 63, 64);
@@ -124,6 +124,7 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
                 rt_ey = y0 - y1;
                 if (z1 != z0) {
                     const float t = (-g.det_r - z0) / (z1 - z0);
+                    rt_t = t;
                     const float y2 = fmaf(y1 - y0, t, y0);
                     tap((y2 - g.y_d0) / g.dby, rt_jl, rt_fj);
                 }
@@ -137,7 +138,9 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
             if (z1 == z0) continue;
             const float ex = x0 - x1, ez = z0 - z1;
             const float exx = ex * ex;
-            const float t = (-g.det_r - z0) / (z1 - z0);
+            // the view's ray parameter (the slice's, the same for every lane):
+            // row 0's row-term lane divided it already
+            const float t = readlane_f(rt_t, k * ROWS);
             const float x2 = fmaf(x1 - x0, t, x0);
             int il;
             float fi;
