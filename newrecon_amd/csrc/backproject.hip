@@ -131,6 +131,10 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
             }
         }
         const int na = n - c0 < VIEW_CHUNK ? n - c0 : VIEW_CHUNK;
+        // y0 - y1 of row r does not depend on the view: read once per chunk
+        float eyr[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) eyr[r] = readlane_f(rt_ey, r);
         for (int k = 0; k < na; ++k) {
             const int a = c0 + k;
             const float x0 = readlane_f(sx0, a), z0 = readlane_f(sz0, a);
@@ -190,7 +194,7 @@ __device__ __forceinline__ void sbp_rows(const DevGeom& g, const SlotTab& S, int
                 for (int r = 0; r < ROWS; ++r) {
                     const int src = k * ROWS + r;
                     const float fj = readlane_f(rt_fj, src);
-                    const float ey = readlane_f(rt_ey, src);
+                    const float ey = eyr[r];
                     const float den = fmaf(ez, ez, fmaf(ey, ey, exx));
                     const float dw = num * __builtin_amdgcn_rcpf(den);
                     float d1 = omfi * p[r][0];
